@@ -1,0 +1,206 @@
+/*
+ * kate_hip.h -- C ABI of the MI355X state-vector library (libkate_hip.so).
+ *
+ * Kate (the reference) exposes only a header-only C++ API; it has no C ABI,
+ * plugin registry or FFI.  Every entry point below replaces one step of the
+ * reference's single-GPU simulate call, cited per declaration.  The C++
+ * drop-in layer (kate_amd/kate/Kate.hpp) lowers Kate's QuantumCircuit onto
+ * these calls; ctypes (kate_amd/__init__.py) binds the same symbols.
+ *
+ * Conventions: plain pointers and sizes, int status (KATE_OK = 0, negative on
+ * error, message in kate_last_error()), no exceptions across the boundary.
+ * Amplitudes are interleaved complex (re, im): complex128 (16 B) or complex64
+ * (8 B).  Bit q of an amplitude index is the value of qubit q.
+ */
+#ifndef KATE_HIP_H
+#define KATE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KATE_MAX_QUBITS 40
+#define KATE_MAX_GATE_QUBITS 16
+
+enum kate_status {
+    KATE_OK = 0,
+    KATE_ERR_ARG = -1,
+    KATE_ERR_NO_DEVICE = -2,
+    KATE_ERR_HIP = -3,
+    KATE_ERR_OOM = -4,
+    KATE_ERR_PROGRAM = -5,
+    KATE_ERR_UNSUPPORTED = -6,
+    KATE_ERR_BUSY = -7
+};
+
+enum kate_dtype { KATE_C128 = 0, KATE_C64 = 1 };
+
+/* Gate ids: backendagnosticcode/agnosticpreprocessor.hpp:28-33. */
+enum kate_gate_id { KATE_DENSE = 0, KATE_H = 2, KATE_CNOT = 3, KATE_CRK = 4, KATE_TOFFOLI = 5 };
+
+/* One gate of the flat program: the lowering of Kate::Gate
+ * (backendagnosticcode/basic_host_types.hpp:190-296).  qubits[i] is the
+ * qubit of matrix-index bit i (order is semantic); matrix_offset counts
+ * complex elements into kate_program.matrix_pool (-1 when not dense). */
+typedef struct kate_gate_rec {
+    int32_t id;
+    int32_t nq;
+    int32_t qubits[KATE_MAX_GATE_QUBITS];
+    int32_t optarg;
+    int32_t reserved;
+    int64_t matrix_offset;
+} kate_gate_rec;
+
+/* The flat lowering of Kate::QuantumCircuit (QuantumCircuit.hpp:12-17):
+ * gates in execution order; groups[g] = (end-exclusive gate index, qubit
+ * bitmask); instruction i is a compute (kind 1, "run the next group") or a
+ * swap list (kind 0) whose position pairs are
+ * pairs[instr_pair_offset[i] .. + 2*instr_npairs[i]); permutations as Kate
+ * stores them (initial: real -> virtual, final inverse: virtual -> real).
+ * NULL permutations mean identity; ninstr == 0 means "one compute per group".
+ * The library copies everything it needs; the caller keeps ownership. */
+typedef struct kate_program {
+    int32_t nqubits;
+    int32_t ngates;
+    const kate_gate_rec* gates;
+    const double* matrix_pool; /* interleaved complex128 */
+    int64_t matrix_pool_len;   /* complex elements */
+    int32_t ngroups;
+    const int32_t* group_end;
+    const uint64_t* group_mask;
+    int32_t ninstr;
+    const int32_t* instr_kind;
+    const int32_t* instr_npairs;
+    const int32_t* instr_pair_offset;
+    const int32_t* pairs;
+    const int32_t* initial_permutation;
+    const int32_t* final_inverse_permutation;
+} kate_program;
+
+/* Execution options (no counterpart in Kate, whose kernels have one shape). */
+typedef struct kate_options {
+    int32_t dtype;        /* kate_dtype; KATE_C128 is the plain path */
+    int32_t tile_qubits;  /* 0 = default per dtype (12 for C128, 13 for C64) */
+    int32_t low_qubits;   /* qubits always in a tile (coalesced run = 2^low amplitudes); 0 = default */
+    int32_t use_mfma;     /* dense gates of >= 4 qubits on matrix cores (C64 only) */
+    int32_t fold_remaps;  /* reserved, 0 */
+    int32_t device;       /* HIP device ordinal */
+    int32_t reserved[2];
+} kate_options;
+
+/* Wall-clock split of the last execute, as Kate's execute(true) prints it
+ * (HIPcode/simulator.hpp:301-353), plus the pass statistics of the plan. */
+typedef struct kate_timings {
+    double init_ms;
+    double compute_ms;
+    double measure_ms;
+    int32_t passes;        /* gate-application passes over the state */
+    int32_t remap_passes;  /* qubit-remap passes (Kate's swap instructions) */
+    int32_t micro_steps;   /* register micro-steps inside tiles, summed over passes */
+    int32_t gates;         /* gates in the program */
+    double bytes_per_pass; /* 2 * 2^n * sizeof(amplitude) */
+} kate_timings;
+
+typedef struct kate_sim kate_sim;
+typedef struct kate_circuit kate_circuit;
+
+/* ---- library ---- */
+const char* kate_status_string(int status);
+/* Thread-local message of the last failing call on this thread. */
+const char* kate_last_error(void);
+/* HIPcode/simulator.hpp:244-259 (hipGetDeviceCount).  No CPU fallback. */
+int kate_device_count(int* count);
+/* HIPcode/DeviceInfo.hpp:10-36 (printGpuInfo): one line per device into buf. */
+int kate_device_info(char* buf, size_t len);
+
+/* ---- simulator: HIPcode/simulator.hpp:220-543,760-810 ---- */
+/* Simulator::Simulator (HIPcode/simulator.hpp:243-300): validates and copies
+ * the program, plans its passes, allocates the state (hipMalloc of 2^n
+ * amplitudes) and uploads the whole program once. */
+int kate_sim_create(const kate_program* program, const kate_options* options, kate_sim** out);
+/* Simulator::~Simulator (HIPcode/simulator.hpp:407-425). */
+int kate_sim_destroy(kate_sim* sim);
+/* initialize() / initialize_state (HIPcode/simulator.hpp:427-443,17-24),
+ * generalised to any basis index (virtual order). */
+int kate_sim_init_basis(kate_sim* sim, uint64_t index);
+/* initialize(proba_state&) / initialize_probastate (HIPcode/simulator.hpp:444-482,26-38):
+ * theta_phi[2*i], theta_phi[2*i+1] for real qubit i. */
+int kate_sim_init_product(kate_sim* sim, const double* theta_phi);
+/* The instruction loop of execute() (HIPcode/simulator.hpp:321-339): every
+ * compute group and swap instruction, enqueued on the simulator's stream. */
+int kate_sim_run(kate_sim* sim);
+/* measurement() (HIPcode/simulator.hpp:483-543): Kate's proba_state,
+ * theta_phi[2*q], theta_phi[2*q+1] for real qubit q. */
+int kate_sim_measure_proba(kate_sim* sim, double* theta_phi);
+/* execute(displaytime) / execute(proba_state&, displaytime)
+ * (HIPcode/simulator.hpp:301-406): init (basis |0...0> when theta_phi_in is
+ * NULL, else product state) + run + measurement, timed like Kate's. */
+int kate_sim_execute(kate_sim* sim, const double* theta_phi_in, double* theta_phi_out);
+/* No counterpart in Kate: <Z_S> = sum_x |a_x|^2 (-1)^popcount(x & S) for each
+ * real-qubit mask S, accumulated in FP64, normalised by the state's norm. */
+int kate_sim_expect_z(kate_sim* sim, const uint64_t* masks, int32_t count, double* out);
+/* No counterpart in Kate: sum_x |a_x|^2 in FP64. */
+int kate_sim_norm(kate_sim* sim, double* out);
+/* No counterpart in Kate: rescale the state to norm 1 (factor in FP64). */
+int kate_sim_renormalize(kate_sim* sim);
+/* Public member Simulator::gpu_qbits_states (HIPcode/simulator.hpp:232): the
+ * amplitudes, copied to host memory, in virtual order (real_order = 0, as
+ * Kate leaves them) or real order (real_order = 1, final_inverse_permutation
+ * applied).  bytes must be 2^n * sizeof(amplitude). */
+int kate_sim_read_state(kate_sim* sim, void* host, uint64_t bytes, int32_t real_order);
+/* No counterpart in Kate: overwrite the state (virtual order). */
+int kate_sim_write_state(kate_sim* sim, const void* host, uint64_t bytes);
+/* Device address of the state (Simulator::gpu_qbits_states[0]). */
+int kate_sim_device_state(kate_sim* sim, void** dptr);
+int kate_sim_last_timings(kate_sim* sim, kate_timings* out);
+int kate_sim_synchronize(kate_sim* sim);
+/* HIP stream the simulator enqueues on (hipStream_t, as void*). */
+int kate_sim_stream(kate_sim* sim, void** stream);
+/* Pass plan as text (one line per pass), for inspection and tests. */
+int kate_sim_describe(kate_sim* sim, char* buf, size_t len);
+
+/* ---- circuit compiler: backendagnosticcode/Circuit.hpp, QuantumCircuit.hpp ---- */
+/* Circuit(int nqbits) (QuantumCircuit.hpp:18-20). */
+int kate_circuit_create(int32_t nqubits, kate_circuit** out);
+int kate_circuit_destroy(kate_circuit* c);
+/* appendGate (QuantumCircuit.hpp:25-37) of Gate(id, qbits, optarg)
+ * (basic_host_types.hpp:198-228) or, for id 0, Gate(Matrix, qbits)
+ * (:229-245); matrix is 2^nq x 2^nq interleaved complex128, row-major. */
+int kate_circuit_append(kate_circuit* c, int32_t id, int32_t nq, const int32_t* qubits, int32_t optarg,
+                        const double* matrix);
+/* Circuit::gateScheduling (Circuit.hpp:12-98). */
+int kate_circuit_schedule(kate_circuit* c);
+/* Circuit::gateFusion (Circuit.hpp:99-175) with CPUmergeGate (CPUcode/GateMerger.hpp:8-55). */
+int kate_circuit_fuse(kate_circuit* c, int32_t max_qubits, double merge_time_matters);
+/* Circuit::gateGrouping (Circuit.hpp:176-206). */
+int kate_circuit_group(kate_circuit* c, int32_t group_qubits);
+/* Circuit::allocate (Circuit.hpp:207-348). */
+int kate_circuit_allocate(kate_circuit* c, int32_t gpu_log2, int32_t max_local);
+/* QuantumCircuit::compileDefault (QuantumCircuit.hpp:126-204). */
+int kate_circuit_compile_default(kate_circuit* c, int32_t gpu_log2, int32_t max_local);
+/* Circuit::compileOPT (Circuit.hpp:349-354). */
+int kate_circuit_compile_opt(kate_circuit* c, int32_t fuse_qubits, double merge_time_matters, int32_t group_qubits,
+                             int32_t gpu_log2, int32_t max_local);
+/* Circuit::exportqcx (Circuit.hpp:763-774). */
+int kate_circuit_export_qcx(kate_circuit* c, const char* path);
+/* Read back: counts, gate i, groups, instructions, permutations. */
+int kate_circuit_info(const kate_circuit* c, int32_t* nqubits, int32_t* ngates, int32_t* ngroups, int32_t* ninstr);
+int kate_circuit_gate(const kate_circuit* c, int32_t i, int32_t* id, int32_t* nq, int32_t* qubits, int32_t* optarg,
+                      double* matrix /* may be NULL */);
+int kate_circuit_groups(const kate_circuit* c, int32_t* ends, uint64_t* masks);
+/* Instruction i: kind, and up to cap ints of its pair list; returns the
+ * pair-list length through *npairs_ints. */
+int kate_circuit_instruction(const kate_circuit* c, int32_t i, int32_t* kind, int32_t* pairs, int32_t cap,
+                             int32_t* npairs_ints);
+int kate_circuit_permutations(const kate_circuit* c, int32_t* initial, int32_t* final_inverse);
+/* Simulator(QuantumCircuit, 1) on this circuit (compiled with
+ * compileDefault(0) first if it has no instructions, HIPcode/simulator.hpp:261-264). */
+int kate_sim_create_from_circuit(const kate_circuit* c, const kate_options* options, kate_sim** out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KATE_HIP_H */

@@ -1,0 +1,344 @@
+"""kate_amd -- MI355X state-vector simulator with Kate's API (Python binding).
+
+ctypes over the C ABI of ``libkate_hip.so`` (include/kate_hip.h).  The
+classes mirror Kate's C++ API: ``Circuit`` (backendagnosticcode/Circuit.hpp:
+gateScheduling / gateFusion / gateGrouping / allocate / compileDefault) and
+``Simulator`` (HIPcode/simulator.hpp: execute, proba_state).  There is no CPU
+path: without the built library or a GPU every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Iterable, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from .workloads import Gate  # noqa: F401  (re-exported)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libkate_hip.so")
+
+KATE_C128, KATE_C64 = 0, 1
+MAX_GATE_QUBITS = 16
+
+_lib = None
+
+
+class KateError(RuntimeError):
+    pass
+
+
+class _GateRec(C.Structure):
+    _fields_ = [("id", C.c_int32), ("nq", C.c_int32), ("qubits", C.c_int32 * MAX_GATE_QUBITS),
+                ("optarg", C.c_int32), ("reserved", C.c_int32), ("matrix_offset", C.c_int64)]
+
+
+class Options(C.Structure):
+    _fields_ = [("dtype", C.c_int32), ("tile_qubits", C.c_int32), ("low_qubits", C.c_int32),
+                ("use_mfma", C.c_int32), ("fold_remaps", C.c_int32), ("device", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
+
+
+class Timings(C.Structure):
+    _fields_ = [("init_ms", C.c_double), ("compute_ms", C.c_double), ("measure_ms", C.c_double),
+                ("passes", C.c_int32), ("remap_passes", C.c_int32), ("micro_steps", C.c_int32),
+                ("gates", C.c_int32), ("bytes_per_pass", C.c_double)]
+
+
+EXPORTS = [
+    "kate_status_string", "kate_last_error", "kate_device_count", "kate_device_info",
+    "kate_sim_create", "kate_sim_destroy", "kate_sim_init_basis", "kate_sim_init_product", "kate_sim_run",
+    "kate_sim_measure_proba", "kate_sim_execute", "kate_sim_expect_z", "kate_sim_norm", "kate_sim_renormalize",
+    "kate_sim_read_state", "kate_sim_write_state", "kate_sim_device_state", "kate_sim_last_timings",
+    "kate_sim_synchronize", "kate_sim_stream", "kate_sim_describe",
+    "kate_circuit_create", "kate_circuit_destroy", "kate_circuit_append", "kate_circuit_schedule",
+    "kate_circuit_fuse", "kate_circuit_group", "kate_circuit_allocate", "kate_circuit_compile_default",
+    "kate_circuit_compile_opt", "kate_circuit_export_qcx", "kate_circuit_info", "kate_circuit_gate",
+    "kate_circuit_groups", "kate_circuit_instruction", "kate_circuit_permutations", "kate_sim_create_from_circuit",
+]
+
+VP = C.c_void_p
+I32P = C.POINTER(C.c_int32)
+U64P = C.POINTER(C.c_uint64)
+DP = C.POINTER(C.c_double)
+
+
+def lib():
+    """Load libkate_hip.so (raises KateError when it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise KateError(f"{LIB_PATH} is not built; run __graft_entry__.build() or make -C kate_amd")
+        L = C.CDLL(LIB_PATH)
+        L.kate_status_string.restype = C.c_char_p
+        L.kate_status_string.argtypes = [C.c_int]
+        L.kate_last_error.restype = C.c_char_p
+        L.kate_device_count.argtypes = [I32P]
+        L.kate_device_info.argtypes = [C.c_char_p, C.c_size_t]
+        L.kate_sim_create.argtypes = [VP, C.POINTER(Options), C.POINTER(VP)]
+        L.kate_sim_destroy.argtypes = [VP]
+        L.kate_sim_init_basis.argtypes = [VP, C.c_uint64]
+        L.kate_sim_init_product.argtypes = [VP, DP]
+        L.kate_sim_run.argtypes = [VP]
+        L.kate_sim_measure_proba.argtypes = [VP, DP]
+        L.kate_sim_execute.argtypes = [VP, DP, DP]
+        L.kate_sim_expect_z.argtypes = [VP, U64P, C.c_int32, DP]
+        L.kate_sim_norm.argtypes = [VP, DP]
+        L.kate_sim_renormalize.argtypes = [VP]
+        L.kate_sim_read_state.argtypes = [VP, VP, C.c_uint64, C.c_int32]
+        L.kate_sim_write_state.argtypes = [VP, VP, C.c_uint64]
+        L.kate_sim_device_state.argtypes = [VP, C.POINTER(VP)]
+        L.kate_sim_last_timings.argtypes = [VP, C.POINTER(Timings)]
+        L.kate_sim_synchronize.argtypes = [VP]
+        L.kate_sim_stream.argtypes = [VP, C.POINTER(VP)]
+        L.kate_sim_describe.argtypes = [VP, C.c_char_p, C.c_size_t]
+        L.kate_circuit_create.argtypes = [C.c_int32, C.POINTER(VP)]
+        L.kate_circuit_destroy.argtypes = [VP]
+        L.kate_circuit_append.argtypes = [VP, C.c_int32, C.c_int32, I32P, C.c_int32, DP]
+        L.kate_circuit_schedule.argtypes = [VP]
+        L.kate_circuit_fuse.argtypes = [VP, C.c_int32, C.c_double]
+        L.kate_circuit_group.argtypes = [VP, C.c_int32]
+        L.kate_circuit_allocate.argtypes = [VP, C.c_int32, C.c_int32]
+        L.kate_circuit_compile_default.argtypes = [VP, C.c_int32, C.c_int32]
+        L.kate_circuit_compile_opt.argtypes = [VP, C.c_int32, C.c_double, C.c_int32, C.c_int32, C.c_int32]
+        L.kate_circuit_export_qcx.argtypes = [VP, C.c_char_p]
+        L.kate_circuit_info.argtypes = [VP, I32P, I32P, I32P, I32P]
+        L.kate_circuit_gate.argtypes = [VP, C.c_int32, I32P, I32P, I32P, I32P, DP]
+        L.kate_circuit_groups.argtypes = [VP, I32P, U64P]
+        L.kate_circuit_instruction.argtypes = [VP, C.c_int32, I32P, I32P, C.c_int32, I32P]
+        L.kate_circuit_permutations.argtypes = [VP, I32P, I32P]
+        L.kate_sim_create_from_circuit.argtypes = [VP, C.POINTER(Options), C.POINTER(VP)]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        msg = lib().kate_last_error().decode(errors="replace")
+        raise KateError(f"{what}: {lib().kate_status_string(rc).decode()} ({msg})")
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def device_count() -> int:
+    n = C.c_int32(0)
+    rc = lib().kate_device_count(C.byref(n))
+    return int(n.value) if rc == 0 else 0
+
+
+def device_info() -> str:
+    buf = C.create_string_buffer(4096)
+    _check(lib().kate_device_info(buf, 4096), "kate_device_info")
+    return buf.value.decode()
+
+
+class Circuit:
+    """Kate::Circuit through the C ABI: build, run the passes, read back."""
+
+    def __init__(self, nqubits: int, gates: Iterable = ()):
+        self.n = nqubits
+        h = VP()
+        _check(lib().kate_circuit_create(nqubits, C.byref(h)), "kate_circuit_create")
+        self._h = h
+        for g in gates:
+            self.append(g)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.kate_circuit_destroy(h)
+            self._h = None
+
+    def append(self, g) -> "Circuit":
+        qs = np.array(g.qubits, dtype=np.int32)
+        mat = None
+        if g.id == 0:
+            mat = np.ascontiguousarray(g.mat, dtype=np.complex128).view(np.float64).ravel()
+        _check(lib().kate_circuit_append(self._h, g.id, len(qs), _p(qs, I32P), g.optarg,
+                                         _p(mat, DP) if mat is not None else None), "kate_circuit_append")
+        return self
+
+    def extend(self, gates: Iterable) -> "Circuit":
+        for g in gates:
+            self.append(g)
+        return self
+
+    # passes (backendagnosticcode/Circuit.hpp)
+    def schedule(self):
+        _check(lib().kate_circuit_schedule(self._h), "gateScheduling")
+        return self
+
+    def fuse(self, max_qubits: int = 5, merge_time_matters: float = 1e-6):
+        _check(lib().kate_circuit_fuse(self._h, max_qubits, merge_time_matters), "gateFusion")
+        return self
+
+    def group(self, group_qubits: int = 8):
+        _check(lib().kate_circuit_group(self._h, group_qubits), "gateGrouping")
+        return self
+
+    def allocate(self, gpu_log2: int = 0, max_local: int = 300):
+        _check(lib().kate_circuit_allocate(self._h, gpu_log2, max_local), "allocate")
+        return self
+
+    def compile_default(self, gpu_log2: int = 0, max_local: int = 300):
+        _check(lib().kate_circuit_compile_default(self._h, gpu_log2, max_local), "compileDefault")
+        return self
+
+    def compile_opt(self, fuse_qubits=5, merge_time_matters=1e-6, group_qubits=10, gpu_log2=0, max_local=300):
+        _check(lib().kate_circuit_compile_opt(self._h, fuse_qubits, merge_time_matters, group_qubits, gpu_log2,
+                                              max_local), "compileOPT")
+        return self
+
+    def export_qcx(self, path: str):
+        _check(lib().kate_circuit_export_qcx(self._h, path.encode()), "exportqcx")
+
+    def info(self) -> Tuple[int, int, int, int]:
+        a, b, c, d = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        _check(lib().kate_circuit_info(self._h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)), "info")
+        return a.value, b.value, c.value, d.value
+
+    def gates(self) -> List[Gate]:
+        out = []
+        _, ng, _, _ = self.info()
+        idv, nq, oa = C.c_int32(), C.c_int32(), C.c_int32()
+        qs = np.zeros(MAX_GATE_QUBITS, dtype=np.int32)
+        for i in range(ng):
+            _check(lib().kate_circuit_gate(self._h, i, C.byref(idv), C.byref(nq), _p(qs, I32P), C.byref(oa), None),
+                   "gate")
+            k = nq.value
+            mat = None
+            if idv.value == 0:
+                m = np.zeros(2 * (1 << (2 * k)), dtype=np.float64)
+                _check(lib().kate_circuit_gate(self._h, i, C.byref(idv), C.byref(nq), _p(qs, I32P), C.byref(oa),
+                                               _p(m, DP)), "gate")
+                mat = m.view(np.complex128).reshape(1 << k, 1 << k)
+            out.append(Gate(idv.value, tuple(int(x) for x in qs[:k]), oa.value, mat))
+        return out
+
+    def groups(self) -> List[Tuple[int, int]]:
+        _, _, ngr, _ = self.info()
+        e = np.zeros(max(1, ngr), dtype=np.int32)
+        m = np.zeros(max(1, ngr), dtype=np.uint64)
+        _check(lib().kate_circuit_groups(self._h, _p(e, I32P), _p(m, U64P)), "groups")
+        return [(int(e[i]), int(m[i])) for i in range(ngr)]
+
+    def instructions(self) -> List[Tuple[int, List[int]]]:
+        _, _, _, ni = self.info()
+        out = []
+        kind, cnt = C.c_int32(), C.c_int32()
+        buf = np.zeros(256, dtype=np.int32)
+        for i in range(ni):
+            _check(lib().kate_circuit_instruction(self._h, i, C.byref(kind), _p(buf, I32P), 256, C.byref(cnt)),
+                   "instruction")
+            out.append((kind.value, [int(x) for x in buf[:cnt.value]]))
+        return out
+
+    def permutations(self) -> Tuple[List[int], List[int]]:
+        a = np.zeros(self.n, dtype=np.int32)
+        b = np.zeros(self.n, dtype=np.int32)
+        _check(lib().kate_circuit_permutations(self._h, _p(a, I32P), _p(b, I32P)), "permutations")
+        ip, fi = [int(x) for x in a], [int(x) for x in b]
+        _, _, _, ni = self.info()
+        if ni == 0:
+            return [], []
+        return ip, fi
+
+
+class Simulator:
+    """Kate::Simulator on one GPU (HIPcode/simulator.hpp:220-543)."""
+
+    def __init__(self, circuit: Circuit, dtype: str = "c128", tile_qubits: int = 0, low_qubits: int = 0,
+                 device: int = 0):
+        self.n = circuit.n
+        self.dtype = {"c128": KATE_C128, "c64": KATE_C64}[dtype]
+        self.amp = np.complex128 if self.dtype == KATE_C128 else np.complex64
+        opt = Options(self.dtype, tile_qubits, low_qubits, 0, 0, device)
+        h = VP()
+        _check(lib().kate_sim_create_from_circuit(circuit._h, C.byref(opt), C.byref(h)), "kate_sim_create")
+        self._h = h
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.kate_sim_destroy(h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def execute(self, proba_in: Optional[np.ndarray] = None) -> np.ndarray:
+        """Kate's execute(): init (|0..0> or product state), run, proba_state (n, 2)."""
+        out = np.zeros(2 * self.n)
+        pin = None if proba_in is None else np.ascontiguousarray(proba_in, dtype=np.float64).ravel()
+        _check(lib().kate_sim_execute(self._h, _p(pin, DP) if pin is not None else None, _p(out, DP)), "execute")
+        return out.reshape(self.n, 2)
+
+    def init_basis(self, index: int = 0):
+        _check(lib().kate_sim_init_basis(self._h, index), "init_basis")
+
+    def init_product(self, theta_phi: np.ndarray):
+        tp = np.ascontiguousarray(theta_phi, dtype=np.float64).ravel()
+        _check(lib().kate_sim_init_product(self._h, _p(tp, DP)), "init_product")
+
+    def run(self):
+        _check(lib().kate_sim_run(self._h), "run")
+
+    def synchronize(self):
+        _check(lib().kate_sim_synchronize(self._h), "synchronize")
+
+    def measure_proba(self) -> np.ndarray:
+        out = np.zeros(2 * self.n)
+        _check(lib().kate_sim_measure_proba(self._h, _p(out, DP)), "measure")
+        return out.reshape(self.n, 2)
+
+    def expect_z(self, masks: Sequence[int]) -> np.ndarray:
+        m = np.array(masks, dtype=np.uint64)
+        out = np.zeros(len(m))
+        _check(lib().kate_sim_expect_z(self._h, _p(m, U64P), len(m), _p(out, DP)), "expect_z")
+        return out
+
+    def norm(self) -> float:
+        v = C.c_double()
+        _check(lib().kate_sim_norm(self._h, C.byref(v)), "norm")
+        return v.value
+
+    def renormalize(self):
+        _check(lib().kate_sim_renormalize(self._h), "renormalize")
+
+    def state(self, real_order: bool = True) -> np.ndarray:
+        out = np.zeros(1 << self.n, dtype=self.amp)
+        _check(lib().kate_sim_read_state(self._h, out.ctypes.data, out.nbytes, 1 if real_order else 0), "read_state")
+        return out
+
+    def write_state(self, st: np.ndarray):
+        a = np.ascontiguousarray(st, dtype=self.amp)
+        _check(lib().kate_sim_write_state(self._h, a.ctypes.data, a.nbytes), "write_state")
+
+    def device_state(self) -> int:
+        p = VP()
+        _check(lib().kate_sim_device_state(self._h, C.byref(p)), "device_state")
+        return int(p.value or 0)
+
+    def stream(self) -> int:
+        p = VP()
+        _check(lib().kate_sim_stream(self._h, C.byref(p)), "stream")
+        return int(p.value or 0)
+
+    def timings(self) -> dict:
+        t = Timings()
+        _check(lib().kate_sim_last_timings(self._h, C.byref(t)), "timings")
+        return {f: getattr(t, f) for f, _ in Timings._fields_}
+
+    def describe(self) -> str:
+        buf = C.create_string_buffer(1 << 20)
+        _check(lib().kate_sim_describe(self._h, buf, 1 << 20), "describe")
+        return buf.value.decode()

@@ -1,0 +1,613 @@
+// kernels.hip -- gfx950 kernels of the state-vector library.
+//
+// k_pass is the gate-group kernel: the replacement of Kate's
+// executeGroupKernelSharedState + GPUGate::compute
+// (HIPcode/simulator.hpp:143-218, HIPcode/GPUGate.hpp:21-203) and, with a
+// permuted store, of its multi-GPU qubit swap (HIPcode/simulator.hpp:102-115).
+// One workgroup owns one tile of 2^TB amplitudes:
+//   1. load: 16 independent coalesced loads per thread (tile bits 0..TB-5
+//      come from the thread index, so each wave reads whole runs of the
+//      tile's low qubits) into an XOR-swizzled LDS tile;
+//   2. micro-steps: each thread pulls the 16 amplitudes spanned by 4 tile
+//      bits into registers, applies every gate of the step there (gate
+//      coefficients are uniform, read through the scalar cache), writes them
+//      back; one barrier per micro-step;
+//   3. store: the same 16 coalesced runs, optionally to permuted positions.
+// Complex arithmetic follows Kate's Complex class
+// (backendagnosticcode/basic_host_types.hpp:43-45); the compiler may contract
+// it into FMAs, so results agree with Kate's CPU path to rounding, not bits.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace kate {
+namespace {
+
+template <typename R>
+struct alignas(2 * sizeof(R)) Cx {
+    R re, im;
+};
+
+template <typename R>
+__device__ __forceinline__ Cx<R> cmul(Cx<R> a, Cx<R> b)
+{
+    return Cx<R>{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+template <typename R>
+__device__ __forceinline__ Cx<R> cmac(Cx<R> acc, Cx<R> a, Cx<R> b)
+{
+    acc.re = acc.re + a.re * b.re - a.im * b.im;
+    acc.im = acc.im + a.re * b.im + a.im * b.re;
+    return acc;
+}
+
+// XOR swizzle of the low W bits of an LDS element index with the next W bits,
+// W = log2(elements per 256 B bank row): makes the 16-amplitude micro-step
+// accesses of a wave land in distinct banks for every choice of tile bits
+// inside the low 2W bits.
+template <typename R>
+__device__ __forceinline__ uint32_t swz(uint32_t j)
+{
+    constexpr uint32_t W = sizeof(R) == 8 ? 4 : 5;
+    return j ^ ((j >> W) & ((1u << W) - 1u));
+}
+
+// Deposit the low bits of s into the set bits of MASK (compile-time).
+__host__ __device__ constexpr int dep(int mask, int s)
+{
+    int out = 0, k = 0;
+    for (int b = 0; b < 4; b++)
+        if ((mask >> b) & 1) {
+            if ((s >> k) & 1) out |= 1 << b;
+            k++;
+        }
+    return out;
+}
+
+__host__ __device__ constexpr int popc4(int m) { return (m & 1) + ((m >> 1) & 1) + ((m >> 2) & 1) + ((m >> 3) & 1); }
+
+// ---- register ops on the 16 amplitudes of a micro-step -------------------
+
+// Dense gate on the micro bits of MASK; matrix bit i <-> i-th lowest bit.
+template <int MASK, typename R>
+__device__ __forceinline__ void op_dense(Cx<R> (&v)[16], const Cx<R>* __restrict__ m)
+{
+    constexpr int K = popc4(MASK);
+    constexpr int D = 1 << K;
+#pragma unroll
+    for (int rest = 0; rest < 16; rest++) {
+        if (rest & MASK) continue;
+        Cx<R> in[D];
+#pragma unroll
+        for (int s = 0; s < D; s++) in[s] = v[rest | dep(MASK, s)];
+#pragma unroll
+        for (int r = 0; r < D; r++) {
+            Cx<R> acc = cmul(m[r * D], in[0]);
+#pragma unroll
+            for (int c = 1; c < D; c++) acc = cmac(acc, m[r * D + c], in[c]);
+            v[rest | dep(MASK, r)] = acc;
+        }
+    }
+}
+
+template <int J, typename R>
+__device__ __forceinline__ void op_phase(Cx<R> (&v)[16], const Cx<R>* __restrict__ m)
+{
+    const Cx<R> d0 = m[0], d1 = m[1];
+#pragma unroll
+    for (int s = 0; s < 16; s++) v[s] = cmul(((s >> J) & 1) ? d1 : d0, v[s]);
+}
+
+// Kate's Hadamard: a0' = (a0+a1)/sqrt2, a1' = a0' - a1*sqrt2 (GateComputing.hpp:77-78).
+template <int J, typename R>
+__device__ __forceinline__ void op_h(Cx<R> (&v)[16])
+{
+    const R inv = (R)0.7071067811865475, s2 = (R)1.4142135623730951;
+#pragma unroll
+    for (int s = 0; s < 16; s++) {
+        if ((s >> J) & 1) continue;
+        const Cx<R> a = v[s], b = v[s | (1 << J)];
+        const Cx<R> n0{(a.re + b.re) * inv, (a.im + b.im) * inv};
+        v[s] = n0;
+        v[s | (1 << J)] = Cx<R>{n0.re - b.re * s2, n0.im - b.im * s2};
+    }
+}
+
+template <int T, int C, typename R>
+__device__ __forceinline__ void op_cnot(Cx<R> (&v)[16])
+{
+#pragma unroll
+    for (int s = 0; s < 16; s++) {
+        if (((s >> C) & 1) == 0 || ((s >> T) & 1)) continue;
+        const Cx<R> t = v[s];
+        v[s] = v[s | (1 << T)];
+        v[s | (1 << T)] = t;
+    }
+}
+
+template <int A, int B, typename R>
+__device__ __forceinline__ void op_crk(Cx<R> (&v)[16], const Cx<R>* __restrict__ m)
+{
+    const Cx<R> ph = m[0];
+#pragma unroll
+    for (int s = 0; s < 16; s++)
+        if (((s >> A) & 1) && ((s >> B) & 1)) v[s] = cmul(v[s], ph);
+}
+
+template <int T, int C0, int C1, typename R>
+__device__ __forceinline__ void op_toff(Cx<R> (&v)[16])
+{
+#pragma unroll
+    for (int s = 0; s < 16; s++) {
+        if (((s >> C0) & 1) == 0 || ((s >> C1) & 1) == 0 || ((s >> T) & 1)) continue;
+        const Cx<R> t = v[s];
+        v[s] = v[s | (1 << T)];
+        v[s | (1 << T)] = t;
+    }
+}
+
+#define KATE_OPC(kind, var) (((kind) << 8) | (var))
+
+template <typename R>
+__device__ __forceinline__ void apply_op(Cx<R> (&v)[16], uint32_t code, const Cx<R>* __restrict__ cf)
+{
+    switch (code) {
+#define D(M) \
+    case KATE_OPC(OK_DENSE, M): op_dense<M>(v, cf); break;
+        D(1) D(2) D(3) D(4) D(5) D(6) D(7) D(8) D(9) D(10) D(11) D(12) D(13) D(14) D(15)
+#undef D
+#define P(J) \
+    case KATE_OPC(OK_PHASE, J): op_phase<J>(v, cf); break; \
+    case KATE_OPC(OK_H, J): op_h<J>(v); break;
+        P(0) P(1) P(2) P(3)
+#undef P
+#define CN(T, C) \
+    case KATE_OPC(OK_CNOT, (T) * 4 + (C)): op_cnot<T, C>(v); break;
+        CN(0, 1) CN(0, 2) CN(0, 3) CN(1, 0) CN(1, 2) CN(1, 3) CN(2, 0) CN(2, 1) CN(2, 3) CN(3, 0) CN(3, 1) CN(3, 2)
+#undef CN
+#define CR(A, B) \
+    case KATE_OPC(OK_CRK, (A) * 4 + (B)): op_crk<A, B>(v, cf); break;
+        CR(0, 1) CR(0, 2) CR(0, 3) CR(1, 2) CR(1, 3) CR(2, 3)
+#undef CR
+#define TF(T, A, B) \
+    case KATE_OPC(OK_TOFF, (T) * 16 + (A) * 4 + (B)): op_toff<T, A, B>(v); break;
+        TF(0, 1, 2) TF(0, 1, 3) TF(0, 2, 3) TF(1, 0, 2) TF(1, 0, 3) TF(1, 2, 3)
+        TF(2, 0, 1) TF(2, 0, 3) TF(2, 1, 3) TF(3, 0, 1) TF(3, 0, 2) TF(3, 1, 2)
+#undef TF
+    default:
+        break;
+    }
+}
+
+// Index of the `line`-th 16-amplitude group: zero bits inserted at the four
+// ascending tile bits p0 < p1 < p2 < p3.
+__device__ __forceinline__ uint32_t spread4(uint32_t line, uint32_t p0, uint32_t p1, uint32_t p2, uint32_t p3)
+{
+    uint32_t lo;
+    lo = line & ((1u << p0) - 1u); line = lo | ((line - lo) << 1);
+    lo = line & ((1u << p1) - 1u); line = lo | ((line - lo) << 1);
+    lo = line & ((1u << p2) - 1u); line = lo | ((line - lo) << 1);
+    lo = line & ((1u << p3) - 1u); line = lo | ((line - lo) << 1);
+    return line;
+}
+
+// One dense gate of any width on the LDS tile: every thread computes its 16
+// outputs from LDS, then (after a barrier) writes them.
+template <typename R, int TB>
+__device__ __forceinline__ void generic_dense(Cx<R>* tile, const MicroStep& ms, const Cx<R>* __restrict__ m)
+{
+    constexpr int NT = 1 << (TB - 4);
+    const int k = ms.nbits;
+    const uint32_t dim = 1u << k;
+    uint32_t gmask = 0;
+    for (int i = 0; i < k; i++) gmask |= 1u << ms.bits[i];
+    Cx<R> out[16];
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+        const uint32_t j = (uint32_t)e * NT + threadIdx.x;
+        uint32_t row = 0;
+        for (int i = 0; i < k; i++) row |= ((j >> ms.bits[i]) & 1u) << i;
+        const uint32_t base = j & ~gmask;
+        Cx<R> acc{0, 0};
+        for (uint32_t c = 0; c < dim; c++) {
+            uint32_t idx = base;
+            for (int i = 0; i < k; i++) idx |= ((c >> i) & 1u) << ms.bits[i];
+            acc = cmac(acc, m[row * dim + c], tile[swz<R>(idx)]);
+        }
+        out[e] = acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 16; e++) tile[swz<R>((uint32_t)e * NT + threadIdx.x)] = out[e];
+}
+
+// WIDE instantiations also run SK_GENERIC micro-steps; keeping them out of
+// the common variant keeps its register count at two waves per SIMD.
+template <typename R, int TB, bool WIDE>
+__global__ __launch_bounds__(1 << (TB - 4)) void k_pass(Cx<R>* __restrict__ state, PassArgs pa,
+                                                        const MicroStep* __restrict__ micro,
+                                                        const MicroOp* __restrict__ ops,
+                                                        const Cx<R>* __restrict__ coef)
+{
+    constexpr int NT = 1 << (TB - 4);
+    __shared__ Cx<R> tile[1 << TB];
+    const uint32_t tid = threadIdx.x;
+
+    uint64_t base = 0;
+    {
+        const uint64_t b = blockIdx.x;
+        for (int i = 0; i < pa.nrest; i++) base |= ((b >> i) & 1ull) << pa.rest_pos[i];
+    }
+
+    {   // load
+        uint64_t tb = base;
+        for (int b = 0; b < TB - 4; b++) tb |= (uint64_t)((tid >> b) & 1u) << pa.load_pos[b];
+        const uint64_t m0 = 1ull << pa.load_pos[TB - 4], m1 = 1ull << pa.load_pos[TB - 3];
+        const uint64_t m2 = 1ull << pa.load_pos[TB - 2], m3 = 1ull << pa.load_pos[TB - 1];
+        Cx<R> r[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const uint64_t off = tb + ((i & 1) ? m0 : 0) + ((i & 2) ? m1 : 0) + ((i & 4) ? m2 : 0) + ((i & 8) ? m3 : 0);
+            r[i] = state[off];
+        }
+#pragma unroll
+        for (int i = 0; i < 16; i++) tile[swz<R>((uint32_t)i * NT + tid)] = r[i];
+    }
+    __syncthreads();
+
+    for (uint32_t mi = 0; mi < pa.micro_count; mi++) {
+        const MicroStep ms = micro[pa.micro_begin + mi];
+        if (ms.kind == SK_REG) {
+            const uint32_t o0 = 1u << ms.bits[0], o1 = 1u << ms.bits[1], o2 = 1u << ms.bits[2],
+                           o3 = 1u << ms.bits[3];
+            const uint32_t b = spread4(tid, ms.bits[0], ms.bits[1], ms.bits[2], ms.bits[3]);
+            Cx<R> v[16];
+#pragma unroll
+            for (int s = 0; s < 16; s++)
+                v[s] = tile[swz<R>(b | ((s & 1) ? o0 : 0) | ((s & 2) ? o1 : 0) | ((s & 4) ? o2 : 0) | ((s & 8) ? o3 : 0))];
+            for (uint32_t o = 0; o < ms.op_count; o++) {
+                const MicroOp op = ops[ms.op_begin + o];
+                apply_op<R>(v, op.code, coef + op.coef);
+            }
+#pragma unroll
+            for (int s = 0; s < 16; s++)
+                tile[swz<R>(b | ((s & 1) ? o0 : 0) | ((s & 2) ? o1 : 0) | ((s & 4) ? o2 : 0) | ((s & 8) ? o3 : 0))] = v[s];
+        } else if (WIDE) {
+            generic_dense<R, TB>(tile, ms, coef + ms.coef);
+        }
+        __syncthreads();
+    }
+
+    {   // store
+        uint64_t tb = base;
+        for (int b = 0; b < TB - 4; b++) tb |= (uint64_t)((tid >> b) & 1u) << pa.store_pos[b];
+        const uint64_t m0 = 1ull << pa.store_pos[TB - 4], m1 = 1ull << pa.store_pos[TB - 3];
+        const uint64_t m2 = 1ull << pa.store_pos[TB - 2], m3 = 1ull << pa.store_pos[TB - 1];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const uint64_t off = tb + ((i & 1) ? m0 : 0) + ((i & 2) ? m1 : 0) + ((i & 4) ? m2 : 0) + ((i & 8) ? m3 : 0);
+            state[off] = tile[swz<R>((uint32_t)i * NT + tid)];
+        }
+    }
+}
+
+// ---- initialisation --------------------------------------------------------
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_init_basis(Cx<R>* __restrict__ st, uint64_t n, uint64_t index)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        st[i] = Cx<R>{(R)(i == index ? 1 : 0), (R)0};
+}
+
+// Product state, multiplied qubit by qubit in Kate's order
+// (initialize_probastate, HIPcode/simulator.hpp:26-38).
+template <typename R>
+__global__ __launch_bounds__(256) void k_init_product(Cx<R>* __restrict__ st, uint64_t n, int nq,
+                                                       const double* __restrict__ f)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        double re = 1.0, im = 0.0;
+        for (int j = 0; j < nq; j++) {
+            const double* c = f + 2 * (((i >> j) & 1ull) * (uint64_t)nq + (uint64_t)j);
+            const double t = re * c[0] - im * c[1];
+            im = re * c[1] + im * c[0];
+            re = t;
+        }
+        st[i] = Cx<R>{(R)re, (R)im};
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_scale(Cx<R>* __restrict__ st, uint64_t n, double f)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        Cx<R> a = st[i];
+        st[i] = Cx<R>{(R)((double)a.re * f), (R)((double)a.im * f)};
+    }
+}
+
+// ---- deterministic reductions ------------------------------------------------
+
+__device__ __forceinline__ double wave_sum(double x)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+// Sum over the 256 threads of a block, fixed order; result valid in thread 0.
+__device__ __forceinline__ double block_sum(double x, double* lds4)
+{
+    x = wave_sum(x);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) lds4[w] = x;
+    __syncthreads();
+    return ((lds4[0] + lds4[1]) + (lds4[2] + lds4[3]));
+}
+
+// Level 0 of the amplitude sums: a block reduces 4096 amplitudes (tile index
+// it*256 + tid) to one record {total, S1[bit 0..11]}.
+template <typename R>
+__global__ __launch_bounds__(256) void k_sums0(const Cx<R>* __restrict__ st, double* __restrict__ rec)
+{
+    __shared__ double lds[4];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t base = (uint64_t)blockIdx.x << 12;
+    double tr = 0, ti = 0, hr[4] = {0, 0, 0, 0}, hi[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int it = 0; it < 16; it++) {
+        const Cx<R> a = st[base + ((uint64_t)it << 8) + tid];
+        tr += (double)a.re;
+        ti += (double)a.im;
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+            if ((it >> b) & 1) {
+                hr[b] += (double)a.re;
+                hi[b] += (double)a.im;
+            }
+    }
+    double* out = rec + (uint64_t)blockIdx.x * 26;
+    // total
+    double s = block_sum(tr, lds);
+    if (tid == 0) out[0] = s;
+    s = block_sum(ti, lds);
+    if (tid == 0) out[1] = s;
+    // bits 0..7: threads whose index has the bit
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const bool on = (tid >> q) & 1u;
+        s = block_sum(on ? tr : 0.0, lds);
+        if (tid == 0) out[2 + 2 * q] = s;
+        s = block_sum(on ? ti : 0.0, lds);
+        if (tid == 0) out[3 + 2 * q] = s;
+    }
+    // bits 8..11: loop index bits
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        s = block_sum(hr[b], lds);
+        if (tid == 0) out[18 + 2 * b] = s;
+        s = block_sum(hi[b], lds);
+        if (tid == 0) out[19 + 2 * b] = s;
+    }
+}
+
+// Level >= 1: a block reduces 2^C records {total, S1[0..K)} (one per thread)
+// into {total, S1[0..K), S1[K..K+C)}.
+__global__ __launch_bounds__(256) void k_sums_rec(const double* __restrict__ in, int K, int C, double* __restrict__ out)
+{
+    __shared__ double lds[4];
+    const uint32_t tid = threadIdx.x;
+    const bool valid = tid < (1u << C);
+    const uint64_t r = ((uint64_t)blockIdx.x << C) + tid;
+    const int win = 2 * (1 + K), wout = 2 * (1 + K + C);
+    const double* src = in + r * (uint64_t)win;
+    double* dst = out + (uint64_t)blockIdx.x * (uint64_t)wout;
+    const double tr = valid ? src[0] : 0.0, ti = valid ? src[1] : 0.0;
+    for (int f = 0; f < win; f++) {
+        const double s = block_sum(valid ? src[f] : 0.0, lds);
+        if (tid == 0) dst[f] = s;
+    }
+    for (int c = 0; c < C; c++) {
+        const bool on = valid && ((tid >> c) & 1u);
+        double s = block_sum(on ? tr : 0.0, lds);
+        if (tid == 0) dst[win + 2 * c] = s;
+        s = block_sum(on ? ti : 0.0, lds);
+        if (tid == 0) dst[win + 2 * c + 1] = s;
+    }
+}
+
+// Level 0 of the Z sums: a block reduces 4096 amplitudes to MB signed sums.
+template <typename R, int MB>
+__global__ __launch_bounds__(256) void k_z0(const Cx<R>* __restrict__ st, const uint64_t* __restrict__ masks, int m,
+                                            double* __restrict__ part)
+{
+    __shared__ double lds[4];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t base = (uint64_t)blockIdx.x << 12;
+    double acc[MB];
+    uint32_t sgn[MB];
+    uint64_t mk[MB];
+#pragma unroll
+    for (int j = 0; j < MB; j++) {
+        acc[j] = 0.0;
+        mk[j] = j < m ? masks[j] : 0ull;
+        sgn[j] = (uint32_t)(__popcll((base | tid) & mk[j]) & 1);
+    }
+#pragma unroll
+    for (int it = 0; it < 16; it++) {
+        const Cx<R> a = st[base + ((uint64_t)it << 8) + tid];
+        const double p = (double)a.re * (double)a.re + (double)a.im * (double)a.im;
+#pragma unroll
+        for (int j = 0; j < MB; j++) {
+            const uint32_t f = sgn[j] ^ (uint32_t)(__popcll(((uint64_t)it << 8) & mk[j]) & 1);
+            acc[j] += f ? -p : p;
+        }
+    }
+    double* out = part + (uint64_t)blockIdx.x * MB;
+#pragma unroll
+    for (int j = 0; j < MB; j++) {
+        const double s = block_sum(acc[j], lds);
+        if (tid == 0) out[j] = s;
+    }
+}
+
+// rows[count][W] -> out[ceil(count/4096)][W] by fixed-order block sums.
+__global__ __launch_bounds__(256) void k_rowsum(const double* __restrict__ rows, uint64_t count, int W,
+                                                double* __restrict__ out)
+{
+    __shared__ double lds[4];
+    const uint64_t r0 = (uint64_t)blockIdx.x << 12;
+    for (int w = 0; w < W; w++) {
+        double a = 0.0;
+        for (int i = 0; i < 16; i++) {
+            const uint64_t r = r0 + ((uint64_t)i << 8) + threadIdx.x;
+            if (r < count) a += rows[r * W + w];
+        }
+        const double s = block_sum(a, lds);
+        if (threadIdx.x == 0) out[(uint64_t)blockIdx.x * W + w] = s;
+    }
+}
+
+inline unsigned grid_for(uint64_t n)
+{
+    const uint64_t b = (n + 255) / 256;
+    return (unsigned)(b < 65536 ? (b ? b : 1) : 65536);
+}
+
+template <typename R, int TB>
+int pass_t(void* state, int nq_int, const PassArgs& pa, const DevPlan& dp, hipStream_t s)
+{
+    const unsigned grid = 1u << (nq_int - TB);
+    if (pa.has_wide)
+        hipLaunchKernelGGL((k_pass<R, TB, true>), dim3(grid), dim3(1 << (TB - 4)), 0, s, (Cx<R>*)state, pa, dp.micro,
+                           dp.ops, (const Cx<R>*)dp.coef);
+    else
+        hipLaunchKernelGGL((k_pass<R, TB, false>), dim3(grid), dim3(1 << (TB - 4)), 0, s, (Cx<R>*)state, pa, dp.micro,
+                           dp.ops, (const Cx<R>*)dp.coef);
+    return (int)hipGetLastError();
+}
+
+template <typename R>
+int pass_r(int tile_bits, void* state, int nq_int, const PassArgs& pa, const DevPlan& dp, hipStream_t s)
+{
+    switch (tile_bits) {
+    case 10: return pass_t<R, 10>(state, nq_int, pa, dp, s);
+    case 11: return pass_t<R, 11>(state, nq_int, pa, dp, s);
+    case 12: return pass_t<R, 12>(state, nq_int, pa, dp, s);
+    case 13: return pass_t<R, 13>(state, nq_int, pa, dp, s);
+    default: return (int)hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+int launch_pass(int dtype, int tile_bits, void* state, int nq_int, const PassArgs& pa, const DevPlan& dp, void* stream)
+{
+    if (pa.tile_bits != tile_bits || nq_int < tile_bits || nq_int - tile_bits > 31) return (int)hipErrorInvalidValue;
+    if (dtype == 0) return pass_r<double>(tile_bits, state, nq_int, pa, dp, (hipStream_t)stream);
+    return pass_r<float>(tile_bits, state, nq_int, pa, dp, (hipStream_t)stream);
+}
+
+int launch_init_basis(int dtype, void* state, int nq_int, uint64_t index, void* stream)
+{
+    const uint64_t n = 1ull << nq_int;
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_init_basis<double>, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream,
+                           (Cx<double>*)state, n, index);
+    else
+        hipLaunchKernelGGL(k_init_basis<float>, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream,
+                           (Cx<float>*)state, n, index);
+    return (int)hipGetLastError();
+}
+
+int launch_init_product(int dtype, void* state, int nq_int, const double* d_factors, void* stream)
+{
+    const uint64_t n = 1ull << nq_int;
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_init_product<double>, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream,
+                           (Cx<double>*)state, n, nq_int, d_factors);
+    else
+        hipLaunchKernelGGL(k_init_product<float>, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream,
+                           (Cx<float>*)state, n, nq_int, d_factors);
+    return (int)hipGetLastError();
+}
+
+int launch_scale(int dtype, void* state, int nq_int, double factor, void* stream)
+{
+    const uint64_t n = 1ull << nq_int;
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_scale<double>, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, (Cx<double>*)state,
+                           n, factor);
+    else
+        hipLaunchKernelGGL(k_scale<float>, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, (Cx<float>*)state, n,
+                           factor);
+    return (int)hipGetLastError();
+}
+
+uint64_t reduce_scratch_doubles(int nq_int)
+{
+    // level-0 records (26 doubles per 4096 amplitudes) twice over, plus the
+    // Z partials (16 per 4096), with slack for the small levels.
+    const uint64_t chunks = 1ull << (nq_int > 12 ? nq_int - 12 : 0);
+    return chunks * 26 * 2 + chunks * 16 * 2 + 4096;
+}
+
+int launch_amp_sums(int dtype, const void* state, int nq_int, double* d_scratch, double* d_out, void* stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (nq_int < 12) return (int)hipErrorInvalidValue;
+    uint64_t count = 1ull << (nq_int - 12);
+    double* a = count == 1 ? d_out : d_scratch;
+    double* b = d_scratch + count * 26 + 64;
+    if (dtype == 0)
+        hipLaunchKernelGGL(k_sums0<double>, dim3((unsigned)count), dim3(256), 0, s, (const Cx<double>*)state, a);
+    else
+        hipLaunchKernelGGL(k_sums0<float>, dim3((unsigned)count), dim3(256), 0, s, (const Cx<float>*)state, a);
+    int K = 12;
+    while (count > 1) {
+        int C = 0;
+        while (C < 8 && (1ull << (C + 1)) <= count) C++;
+        const uint64_t blocks = count >> C;
+        double* dst = (blocks == 1) ? d_out : b;
+        hipLaunchKernelGGL(k_sums_rec, dim3((unsigned)blocks), dim3(256), 0, s, a, K, C, dst);
+        K += C;
+        count = blocks;
+        std::swap(a, b);
+        if (blocks == 1) break;
+    }
+    return (int)hipGetLastError();
+}
+
+int launch_zsums(int dtype, const void* state, int nq_int, const uint64_t* d_masks, int count, double* d_scratch,
+                 double* d_out, void* stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (nq_int < 12 || count < 1 || count > 16) return (int)hipErrorInvalidValue;
+    const uint64_t chunks = 1ull << (nq_int - 12);
+    double* a = chunks == 1 ? d_out : d_scratch;
+    double* b = d_scratch + chunks * 16 + 64;
+    if (dtype == 0)
+        hipLaunchKernelGGL((k_z0<double, 16>), dim3((unsigned)chunks), dim3(256), 0, s, (const Cx<double>*)state,
+                           d_masks, count, a);
+    else
+        hipLaunchKernelGGL((k_z0<float, 16>), dim3((unsigned)chunks), dim3(256), 0, s, (const Cx<float>*)state,
+                           d_masks, count, a);
+    uint64_t rows = chunks;
+    while (rows > 1) {
+        const uint64_t blocks = (rows + 4095) / 4096;
+        double* dst = blocks == 1 ? d_out : b;
+        hipLaunchKernelGGL(k_rowsum, dim3((unsigned)blocks), dim3(256), 0, s, a, rows, 16, dst);
+        rows = blocks;
+        std::swap(a, b);
+        if (blocks == 1) break;
+    }
+    return (int)hipGetLastError();
+}
+
+}  // namespace kate

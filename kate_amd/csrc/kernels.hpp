@@ -1,0 +1,38 @@
+// kernels.hpp -- launchers of the gfx950 kernels (kernels.hip).  Every
+// launcher enqueues on `stream` and returns the hipError_t of the launch as
+// an int (0 = success).
+#pragma once
+
+#include <cstdint>
+
+#include "plan.hpp"
+
+namespace kate {
+
+struct DevPlan {
+    const MicroStep* micro;
+    const MicroOp* ops;
+    const void* coef;  // complex of the state's precision
+};
+
+// One pass (gate application or remap) over the state.
+int launch_pass(int dtype, int tile_bits, void* state, int nq_int, const PassArgs& pa, const DevPlan& dp,
+                void* stream);
+// |index> (basis state).
+int launch_init_basis(int dtype, void* state, int nq_int, uint64_t index, void* stream);
+// Product state: factor[b*nq + j] (complex128) for bit value b of qubit j.
+int launch_init_product(int dtype, void* state, int nq_int, const double* d_factors, void* stream);
+// Multiplies every amplitude by `factor`.
+int launch_scale(int dtype, void* state, int nq_int, double factor, void* stream);
+
+// Sizes of the scratch buffer (in doubles) the reductions need.
+uint64_t reduce_scratch_doubles(int nq_int);
+// Amplitude sums: out[0..1] = sum of all amplitudes, out[2+2q..] = sum of the
+// amplitudes whose bit q is 1, for q < nq_int (complex128, FP64 accumulation,
+// fixed summation order).
+int launch_amp_sums(int dtype, const void* state, int nq_int, double* d_scratch, double* d_out, void* stream);
+// out[j] = sum_x |a_x|^2 (-1)^popcount(x & masks[j]) for j < count (FP64).
+int launch_zsums(int dtype, const void* state, int nq_int, const uint64_t* d_masks, int count, double* d_scratch,
+                 double* d_out, void* stream);
+
+}  // namespace kate

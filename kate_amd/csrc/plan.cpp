@@ -1,0 +1,392 @@
+// plan.cpp -- lowers a Kate program onto passes over the state.
+//
+// Kate runs one kernel per compute group, each a full read+write of the state
+// (HIPcode/simulator.hpp:760-810), with the groups chosen on the host by
+// gateGrouping.  Here the groups only delimit swap instructions: the gates
+// between two swaps are re-packed into as few passes as the tile allows.
+//
+// A pass stages 2^tile_bits amplitudes per workgroup in LDS.  Its tile
+// always contains qubits 0..low-1, so every HBM access is a contiguous run of
+// 2^low amplitudes, plus the qubits its gates need.  Gates may leave program
+// order only past gates they commute with (disjoint qubits): a gate runs in a
+// pass when all earlier gates sharing a qubit with it ran before.  Inside a
+// pass the gates are packed the same way into register micro-steps of 4 tile
+// bits (16 amplitudes per thread, one LDS round trip per micro-step).
+#include "plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <sstream>
+
+namespace kate {
+namespace {
+
+inline int popc(uint64_t x) { return __builtin_popcountll(x); }
+
+// Which gates of `masks` (in order) run when only the qubits in `set` are
+// available: a gate runs if its qubits are in `set` and no earlier gate
+// sharing a qubit with it was left out.  Gates with ok[i] == 0 never run.
+int sweep(const std::vector<uint64_t>& masks, const std::vector<uint8_t>& ok, uint64_t set, uint64_t universe,
+          std::vector<int>* frontier, std::vector<int>* accepted)
+{
+    uint64_t blocked = 0;
+    int count = 0;
+    for (size_t i = 0; i < masks.size(); i++) {
+        const uint64_t m = masks[i];
+        if (m & blocked) {
+            blocked |= m;
+        } else if (ok[i] && (m & ~set) == 0) {
+            count++;
+            if (accepted) accepted->push_back((int)i);
+        } else {
+            if (frontier && ok[i]) frontier->push_back((int)i);
+            blocked |= m;
+        }
+        if ((blocked & universe) == universe) break;
+    }
+    return count;
+}
+
+// Greedy growth of a qubit set up to `capacity`: repeatedly add the new
+// qubits of the ready gate that makes the most gates runnable per added
+// qubit (earliest gate on ties).
+uint64_t choose_set(const std::vector<uint64_t>& masks, const std::vector<uint8_t>& ok, int capacity, uint64_t start,
+                    uint64_t universe)
+{
+    uint64_t set = start;
+    for (;;) {
+        std::vector<int> frontier;
+        const int now = sweep(masks, ok, set, universe, &frontier, nullptr);
+        double best = 0;
+        uint64_t best_new = 0;
+        std::vector<uint64_t> tried;
+        for (int f : frontier) {
+            const uint64_t add = masks[(size_t)f] & ~set;
+            if (popc(set | add) > capacity) continue;
+            if (std::find(tried.begin(), tried.end(), add) != tried.end()) continue;
+            tried.push_back(add);
+            const int gain = sweep(masks, ok, set | add, universe, nullptr, nullptr) - now;
+            const double score = (double)gain / popc(add);
+            if (score > best) {
+                best = score;
+                best_new = add;
+            }
+        }
+        if (!best_new) return set;
+        set |= best_new;
+    }
+}
+
+// Lowest positions below `limit` not in `set`, added until popc == want.
+uint64_t pad_set(uint64_t set, int want, int limit)
+{
+    for (int b = 0; b < limit && popc(set) < want; b++) set |= 1ull << b;
+    return set;
+}
+
+// M'[r'][c'] = M[r][c], where new matrix bit j is old bit order[j].
+std::vector<double> permute_matrix(const double* m, int k, const int* order)
+{
+    const uint64_t dim = 1ull << k;
+    std::vector<double> out(2 * dim * dim);
+    auto remap = [&](uint64_t x) {
+        uint64_t y = 0;
+        for (int j = 0; j < k; j++)
+            if ((x >> j) & 1ull) y |= 1ull << order[j];
+        return y;
+    };
+    for (uint64_t r = 0; r < dim; r++)
+        for (uint64_t c = 0; c < dim; c++) {
+            const uint64_t src = remap(r) * dim + remap(c);
+            out[2 * (r * dim + c)] = m[2 * src];
+            out[2 * (r * dim + c) + 1] = m[2 * src + 1];
+        }
+    return out;
+}
+
+struct Builder {
+    const Program& prog;
+    const PlanOptions& opt;
+    Plan& plan;
+
+    uint32_t add_coef(const double* c, size_t ncomplex)
+    {
+        const uint32_t off = (uint32_t)(plan.coef.size() / 2);
+        plan.coef.insert(plan.coef.end(), c, c + 2 * ncomplex);
+        return off;
+    }
+
+    // Encodes gate g as a register op on the micro bits `bits` (ascending).
+    std::string add_reg_op(const PlanGate& g, const int* tile_of, const uint8_t* bits)
+    {
+        int mi[16];
+        for (int i = 0; i < g.k; i++) {
+            const int tb = tile_of[g.q[i]];
+            mi[i] = -1;
+            for (int j = 0; j < 4; j++)
+                if (bits[j] == tb) mi[i] = j;
+            if (mi[i] < 0) return "internal: gate qubit outside its micro-step";
+        }
+        MicroOp op{};
+        switch (g.id) {
+        case 2: op.code = (uint16_t)(OK_H << 8 | mi[0]); break;
+        case 3: op.code = (uint16_t)(OK_CNOT << 8 | (mi[0] * 4 + mi[1])); break;
+        case 4: {
+            const double ang = ((double)2 * 3.1415926535897932384626433) / (double)(1ull << g.optarg);
+            const double c[2] = {std::cos(ang), std::sin(ang)};
+            op.coef = add_coef(c, 1);
+            op.code = (uint16_t)(OK_CRK << 8 | (std::min(mi[0], mi[1]) * 4 + std::max(mi[0], mi[1])));
+            break;
+        }
+        case 5:
+            op.code = (uint16_t)(OK_TOFF << 8 | (mi[2] * 16 + std::min(mi[0], mi[1]) * 4 + std::max(mi[0], mi[1])));
+            break;
+        case 0: {
+            const double* m = prog.mat.data() + 2 * g.moff;
+            if (g.k == 1 && m[2] == 0.0 && m[3] == 0.0 && m[4] == 0.0 && m[5] == 0.0) {
+                const double d[4] = {m[0], m[1], m[6], m[7]};
+                op.coef = add_coef(d, 2);
+                op.code = (uint16_t)(OK_PHASE << 8 | mi[0]);
+                break;
+            }
+            int order[16];
+            for (int i = 0; i < g.k; i++) order[i] = i;
+            std::sort(order, order + g.k, [&](int a, int b) { return mi[a] < mi[b]; });
+            std::vector<double> pm = permute_matrix(m, g.k, order);
+            op.coef = add_coef(pm.data(), pm.size() / 2);
+            int mask = 0;
+            for (int i = 0; i < g.k; i++) mask |= 1 << mi[i];
+            op.code = (uint16_t)(OK_DENSE << 8 | mask);
+            break;
+        }
+        default:
+            return "unsupported gate id " + std::to_string(g.id);
+        }
+        plan.ops.push_back(op);
+        return "";
+    }
+
+    // Packs the gates of one pass (`gl`, program order) into micro-steps.
+    std::string pack_micro(const std::vector<int>& gl, const int* tile_of, PassArgs& pa)
+    {
+        const int t = plan.tile_bits;
+        std::vector<int> rem = gl;
+        pa.micro_begin = (uint32_t)plan.micro.size();
+        while (!rem.empty()) {
+            std::vector<uint64_t> tm(rem.size());
+            std::vector<uint8_t> ok(rem.size());
+            uint64_t universe = 0;
+            for (size_t i = 0; i < rem.size(); i++) {
+                const PlanGate& g = prog.gates[(size_t)rem[i]];
+                uint64_t m = 0;
+                for (int j = 0; j < g.k; j++) m |= 1ull << tile_of[g.q[j]];
+                tm[i] = m;
+                ok[i] = g.k <= 4 && !(g.id == 0 && opt.mfma && g.k == 4);
+                universe |= m;
+            }
+            const uint64_t set = choose_set(tm, ok, 4, 0, universe);
+            std::vector<int> acc;
+            sweep(tm, ok, set, universe, nullptr, &acc);
+            MicroStep ms{};
+            if (acc.empty()) {
+                // the first remaining gate is one the register path does not take
+                const PlanGate& g = prog.gates[(size_t)rem[0]];
+                if (g.id != 0) return "internal: non-dense gate left out of micro-steps";
+                ms.kind = (opt.mfma && g.k >= 4 && g.k <= 5) ? SK_MFMA : SK_GENERIC;
+                ms.nbits = (uint8_t)g.k;
+                for (int i = 0; i < g.k; i++) ms.bits[i] = (uint8_t)tile_of[g.q[i]];
+                ms.coef = add_coef(prog.mat.data() + 2 * g.moff, (size_t)1 << (2 * g.k));
+                plan.micro.push_back(ms);
+                rem.erase(rem.begin());
+                continue;
+            }
+            const uint64_t full = pad_set(set, 4, t);
+            ms.kind = SK_REG;
+            ms.nbits = 4;
+            int nb = 0;
+            for (int b = 0; b < t; b++)
+                if ((full >> b) & 1ull) ms.bits[nb++] = (uint8_t)b;
+            ms.op_begin = (uint32_t)plan.ops.size();
+            for (int a : acc) {
+                std::string e = add_reg_op(prog.gates[(size_t)rem[(size_t)a]], tile_of, ms.bits);
+                if (!e.empty()) return e;
+            }
+            ms.op_count = (uint32_t)(plan.ops.size() - ms.op_begin);
+            plan.micro.push_back(ms);
+            std::vector<int> next;
+            size_t ai = 0;
+            for (size_t i = 0; i < rem.size(); i++) {
+                if (ai < acc.size() && acc[ai] == (int)i) {
+                    ai++;
+                    continue;
+                }
+                next.push_back(rem[i]);
+            }
+            rem.swap(next);
+        }
+        pa.micro_count = (uint32_t)(plan.micro.size() - pa.micro_begin);
+        pa.has_wide = 0;
+        for (uint32_t m = pa.micro_begin; m < plan.micro.size(); m++)
+            if (plan.micro[m].kind != SK_REG) pa.has_wide = 1;
+        return "";
+    }
+
+    // Fills the position lists of a pass whose tile holds the qubits of `set`.
+    void positions(uint64_t set, PassArgs& pa, int* tile_of)
+    {
+        const int n = plan.nq_int;
+        int tb = 0, rb = 0;
+        for (int q = 0; q < n; q++) {
+            if ((set >> q) & 1ull) {
+                tile_of[q] = tb;
+                pa.load_pos[tb] = pa.store_pos[tb] = (uint8_t)q;
+                tb++;
+            } else {
+                tile_of[q] = -1;
+                pa.rest_pos[rb++] = (uint8_t)q;
+            }
+        }
+        pa.tile_bits = (uint8_t)tb;
+        pa.nrest = (uint8_t)rb;
+    }
+
+    std::string gate_passes(const std::vector<int>& seg)
+    {
+        const int t = plan.tile_bits, n = plan.nq_int;
+        const int low = std::min(opt.low_qubits, t);
+        const uint64_t low_mask = low >= 64 ? ~0ull : ((1ull << low) - 1ull);
+        std::vector<int> rem = seg;
+        while (!rem.empty()) {
+            std::vector<uint64_t> masks(rem.size());
+            std::vector<uint8_t> ok(rem.size(), 1);
+            uint64_t universe = 0;
+            for (size_t i = 0; i < rem.size(); i++) {
+                masks[i] = prog.gates[(size_t)rem[i]].mask;
+                universe |= masks[i];
+            }
+            uint64_t start = low_mask;
+            if (popc(low_mask | masks[0]) > t) start = pad_set(masks[0], std::min(t, popc(masks[0]) + low), n) & ~0ull;
+            if (popc(start) > t) return "gate larger than the tile";
+            const uint64_t set = pad_set(choose_set(masks, ok, t, start, universe), t, n);
+            std::vector<int> acc;
+            sweep(masks, ok, set, universe, nullptr, &acc);
+            if (acc.empty()) return "internal: empty pass";
+            PassArgs pa{};
+            int tile_of[kMaxQubits];
+            positions(set, pa, tile_of);
+            std::vector<int> gl;
+            for (int a : acc) gl.push_back(rem[(size_t)a]);
+            std::string e = pack_micro(gl, tile_of, pa);
+            if (!e.empty()) return e;
+            plan.passes.push_back(pa);
+            plan.pass_is_remap.push_back(0);
+            plan.gate_passes++;
+            std::vector<int> next;
+            size_t ai = 0;
+            for (size_t i = 0; i < rem.size(); i++) {
+                if (ai < acc.size() && acc[ai] == (int)i) {
+                    ai++;
+                    continue;
+                }
+                next.push_back(rem[i]);
+            }
+            rem.swap(next);
+        }
+        return "";
+    }
+
+    // A swap instruction (Kate's swapCommand, HIPcode/simulator.hpp:751-759):
+    // its transpositions of qubit positions, applied in order, become the
+    // store positions of remap passes.
+    std::string remap_passes(const std::vector<int>& pairs)
+    {
+        const int t = plan.tile_bits, n = plan.nq_int;
+        const int low = std::min(opt.low_qubits, t);
+        size_t p = 0;
+        while (p < pairs.size()) {
+            uint64_t set = (1ull << low) - 1ull;
+            size_t end = p;
+            while (end + 1 < pairs.size()) {
+                uint64_t s2 = set | (1ull << pairs[end]) | (1ull << pairs[end + 1]);
+                if (popc(s2) > t) break;
+                set = s2;
+                end += 2;
+            }
+            if (end == p) return "swap pair does not fit a tile";
+            set = pad_set(set, t, n);
+            PassArgs pa{};
+            int tile_of[kMaxQubits];
+            positions(set, pa, tile_of);
+            int where[kMaxQubits];
+            for (int q = 0; q < n; q++) where[q] = q;
+            for (size_t i = p; i < end; i += 2) std::swap(where[pairs[i]], where[pairs[i + 1]]);
+            for (int b = 0; b < pa.tile_bits; b++) pa.store_pos[b] = (uint8_t)where[pa.load_pos[b]];
+            pa.micro_begin = (uint32_t)plan.micro.size();
+            pa.micro_count = 0;
+            plan.passes.push_back(pa);
+            plan.pass_is_remap.push_back(1);
+            plan.remap_passes++;
+            p = end;
+        }
+        return "";
+    }
+};
+
+}  // namespace
+
+std::string build_plan(const Program& prog, const PlanOptions& opt, Plan& plan)
+{
+    plan = Plan();
+    plan.nq_int = std::max(prog.nqubits, opt.min_qubits);
+    if (plan.nq_int > kMaxQubits) return "too many qubits";
+    plan.tile_bits = std::min(opt.tile_bits, plan.nq_int);
+    Builder b{prog, opt, plan};
+    std::vector<int> seg;
+    size_t grp = 0;
+    int first = 0;
+    for (size_t i = 0; i < prog.instr_kind.size(); i++) {
+        if (prog.instr_kind[i] == 0) {
+            if (!seg.empty()) {
+                std::string e = b.gate_passes(seg);
+                if (!e.empty()) return e;
+                seg.clear();
+            }
+            std::string e = b.remap_passes(prog.instr_pairs[i]);
+            if (!e.empty()) return e;
+        } else {
+            if (grp >= prog.group_end.size()) return "more compute instructions than groups";
+            for (int g = first; g < prog.group_end[grp]; g++) seg.push_back(g);
+            first = prog.group_end[grp];
+            grp++;
+        }
+    }
+    if (!seg.empty()) {
+        std::string e = b.gate_passes(seg);
+        if (!e.empty()) return e;
+    }
+    return "";
+}
+
+std::string Plan::describe() const
+{
+    std::ostringstream os;
+    os << "nq_int=" << nq_int << " tile_bits=" << tile_bits << " passes=" << passes.size() << " (gate " << gate_passes
+       << ", remap " << remap_passes << ") micro=" << micro.size() << " ops=" << ops.size() << "\n";
+    for (size_t i = 0; i < passes.size(); i++) {
+        const PassArgs& p = passes[i];
+        os << (pass_is_remap[i] ? "remap" : "pass ") << " " << i << " tile=[";
+        for (int b = 0; b < p.tile_bits; b++) os << (b ? "," : "") << (int)p.load_pos[b];
+        os << "]";
+        if (pass_is_remap[i]) {
+            os << " -> [";
+            for (int b = 0; b < p.tile_bits; b++) os << (b ? "," : "") << (int)p.store_pos[b];
+            os << "]";
+        }
+        size_t nops = 0;
+        for (uint32_t m = 0; m < p.micro_count; m++) nops += micro[p.micro_begin + m].op_count;
+        os << " micro=" << p.micro_count << " ops=" << nops << "\n";
+    }
+    return os.str();
+}
+
+}  // namespace kate

@@ -1,0 +1,108 @@
+// plan.hpp -- host-side program and pass plan shared by the planner and the
+// launchers.  Plain data only (no HIP types), so the planner compiles as
+// ordinary C++ and can be unit-tested on the CPU.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace kate {
+
+// Register micro-step op kinds (op code = kind << 8 | variant).
+enum OpKind : uint16_t {
+    OK_DENSE = 0,  // variant = 4-bit mask of micro bits (ascending = matrix bit order)
+    OK_PHASE = 1,  // diagonal 1-qubit gate, variant = micro bit
+    OK_H = 2,      // variant = micro bit
+    OK_CNOT = 3,   // variant = target*4 + control
+    OK_CRK = 4,    // variant = a*4 + b, a < b
+    OK_TOFF = 5,   // variant = target*16 + c0*4 + c1, c0 < c1
+};
+
+enum StepKind : uint8_t {
+    SK_REG = 0,      // 4 tile bits held in registers, ops applied in place
+    SK_GENERIC = 1,  // one dense gate of any size, matrix-vector on the LDS tile
+    SK_MFMA = 2,     // one dense gate of 4 or 5 qubits on matrix cores (complex64)
+};
+
+struct MicroOp {
+    uint16_t code;
+    uint16_t pad;
+    uint32_t coef;  // complex-element offset into the coefficient pool
+};
+
+struct MicroStep {
+    uint8_t kind;
+    uint8_t nbits;
+    uint8_t pad[2];
+    uint32_t op_begin;
+    uint32_t op_count;
+    uint32_t coef;     // SK_GENERIC/SK_MFMA: matrix offset (complex elements)
+    uint8_t bits[16];  // SK_REG: ascending tile bits; SK_GENERIC/SK_MFMA: tile bit of matrix bit i
+};
+
+constexpr int kMaxQubits = 40;
+
+// One pass over the state: tile bit b of the LDS tile is state qubit
+// load_pos[b] when the tile is read and store_pos[b] when it is written back
+// (they differ only for qubit remaps).  rest_pos lists the other qubits in
+// ascending order; block index bit i is qubit rest_pos[i].
+struct PassArgs {
+    uint8_t tile_bits;
+    uint8_t nrest;
+    uint8_t load_pos[16];
+    uint8_t store_pos[16];
+    uint8_t rest_pos[kMaxQubits];
+    uint8_t has_wide;  // some micro-step is not SK_REG (selects the kernel variant)
+    uint8_t pad;
+    uint32_t micro_begin;
+    uint32_t micro_count;
+};
+
+struct PlanGate {
+    int id = 0;
+    int k = 0;
+    int q[16] = {0};
+    int optarg = 0;
+    int64_t moff = -1;  // complex offset into Program::mat
+    uint64_t mask = 0;
+};
+
+// Validated copy of a kate_program.
+struct Program {
+    int nqubits = 0;
+    std::vector<PlanGate> gates;
+    std::vector<double> mat;  // interleaved complex128
+    std::vector<int> group_end;
+    std::vector<uint64_t> group_mask;
+    // instruction stream: kind 1 = compute next group, kind 0 = swap pairs
+    std::vector<int> instr_kind;
+    std::vector<std::vector<int>> instr_pairs;
+    std::vector<int> initial_permutation;
+    std::vector<int> final_inverse_permutation;
+};
+
+struct PlanOptions {
+    int tile_bits = 12;   // qubits per LDS tile
+    int low_qubits = 3;   // qubits 0..low-1 are in every gate tile
+    int min_qubits = 12;  // the state is padded to at least this many qubits
+    bool mfma = false;    // SK_MFMA for dense gates of 4..5 qubits
+};
+
+struct Plan {
+    int nq_int = 0;  // padded qubit count the kernels see
+    int tile_bits = 0;
+    std::vector<PassArgs> passes;
+    std::vector<uint8_t> pass_is_remap;
+    std::vector<MicroStep> micro;
+    std::vector<MicroOp> ops;
+    std::vector<double> coef;  // interleaved complex128; converted per dtype on upload
+    int gate_passes = 0;
+    int remap_passes = 0;
+    std::string describe() const;
+};
+
+// Builds the pass plan.  Returns an empty string on success, else the error.
+std::string build_plan(const Program& prog, const PlanOptions& opt, Plan& out);
+
+}  // namespace kate
