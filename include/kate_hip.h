@@ -159,8 +159,19 @@ int kate_sim_last_timings(kate_sim* sim, kate_timings* out);
 int kate_sim_synchronize(kate_sim* sim);
 /* HIP stream the simulator enqueues on (hipStream_t, as void*). */
 int kate_sim_stream(kate_sim* sim, void** stream);
+/* kate_sim_run with a HIP event pair around every pass on the simulator's
+ * stream; pass_ms[i] (i < cap) receives the duration of pass i and
+ * pass_is_remap[i] whether it was a qubit remap (either may be NULL).
+ * Returns the number of passes through *npasses.  No counterpart in Kate,
+ * whose execute(true) only prints per-group wall times
+ * (HIPcode/simulator.hpp:322-351). */
+int kate_sim_run_timed(kate_sim* sim, float* pass_ms, int32_t* pass_is_remap, int32_t cap, int32_t* npasses);
 /* Pass plan as text (one line per pass), for inspection and tests. */
 int kate_sim_describe(kate_sim* sim, char* buf, size_t len);
+/* The same plan without a device (no counterpart in Kate, whose grouping is
+ * fixed by gateGrouping): pass and remap counts plus the text form. */
+int kate_plan_describe(const kate_program* program, const kate_options* options, int32_t* passes,
+                       int32_t* remap_passes, char* buf, size_t len);
 
 /* ---- circuit compiler: backendagnosticcode/Circuit.hpp, QuantumCircuit.hpp ---- */
 /* Circuit(int nqbits) (QuantumCircuit.hpp:18-20). */
@@ -199,6 +210,9 @@ int kate_circuit_permutations(const kate_circuit* c, int32_t* initial, int32_t* 
 /* Simulator(QuantumCircuit, 1) on this circuit (compiled with
  * compileDefault(0) first if it has no instructions, HIPcode/simulator.hpp:261-264). */
 int kate_sim_create_from_circuit(const kate_circuit* c, const kate_options* options, kate_sim** out);
+/* kate_plan_describe for this circuit (compiled as kate_sim_create_from_circuit would). */
+int kate_circuit_plan(const kate_circuit* c, const kate_options* options, int32_t* passes, int32_t* remap_passes,
+                      char* buf, size_t len);
 
 #ifdef __cplusplus
 }

@@ -51,11 +51,12 @@ EXPORTS = [
     "kate_sim_create", "kate_sim_destroy", "kate_sim_init_basis", "kate_sim_init_product", "kate_sim_run",
     "kate_sim_measure_proba", "kate_sim_execute", "kate_sim_expect_z", "kate_sim_norm", "kate_sim_renormalize",
     "kate_sim_read_state", "kate_sim_write_state", "kate_sim_device_state", "kate_sim_last_timings",
-    "kate_sim_synchronize", "kate_sim_stream", "kate_sim_describe",
+    "kate_sim_synchronize", "kate_sim_stream", "kate_sim_describe", "kate_plan_describe", "kate_sim_run_timed",
     "kate_circuit_create", "kate_circuit_destroy", "kate_circuit_append", "kate_circuit_schedule",
     "kate_circuit_fuse", "kate_circuit_group", "kate_circuit_allocate", "kate_circuit_compile_default",
     "kate_circuit_compile_opt", "kate_circuit_export_qcx", "kate_circuit_info", "kate_circuit_gate",
     "kate_circuit_groups", "kate_circuit_instruction", "kate_circuit_permutations", "kate_sim_create_from_circuit",
+    "kate_circuit_plan",
 ]
 
 VP = C.c_void_p
@@ -109,6 +110,9 @@ def lib():
         L.kate_circuit_instruction.argtypes = [VP, C.c_int32, I32P, I32P, C.c_int32, I32P]
         L.kate_circuit_permutations.argtypes = [VP, I32P, I32P]
         L.kate_sim_create_from_circuit.argtypes = [VP, C.POINTER(Options), C.POINTER(VP)]
+        L.kate_sim_run_timed.argtypes = [VP, C.POINTER(C.c_float), I32P, C.c_int32, I32P]
+        L.kate_plan_describe.argtypes = [VP, C.POINTER(Options), I32P, I32P, C.c_char_p, C.c_size_t]
+        L.kate_circuit_plan.argtypes = [VP, C.POINTER(Options), I32P, I32P, C.c_char_p, C.c_size_t]
         _lib = L
     return _lib
 
@@ -236,6 +240,14 @@ class Circuit:
             out.append((kind.value, [int(x) for x in buf[:cnt.value]]))
         return out
 
+    def plan(self, dtype: str = "c128", tile_qubits: int = 0, low_qubits: int = 0) -> Tuple[int, int, str]:
+        """(gate passes, remap passes, text) of the device plan, without a GPU."""
+        opt = Options({"c128": KATE_C128, "c64": KATE_C64}[dtype], tile_qubits, low_qubits, 0, 0, 0)
+        p, r = C.c_int32(), C.c_int32()
+        buf = C.create_string_buffer(1 << 20)
+        _check(lib().kate_circuit_plan(self._h, C.byref(opt), C.byref(p), C.byref(r), buf, 1 << 20), "plan")
+        return p.value, r.value, buf.value.decode()
+
     def permutations(self) -> Tuple[List[int], List[int]]:
         a = np.zeros(self.n, dtype=np.int32)
         b = np.zeros(self.n, dtype=np.int32)
@@ -291,6 +303,16 @@ class Simulator:
 
     def run(self):
         _check(lib().kate_sim_run(self._h), "run")
+
+    def run_timed(self) -> Tuple[np.ndarray, np.ndarray]:
+        """Run every pass with HIP events around each; (ms per pass, is-remap flags)."""
+        cap = 1 << 16
+        ms = np.zeros(cap, dtype=np.float32)
+        rm = np.zeros(cap, dtype=np.int32)
+        n = C.c_int32()
+        _check(lib().kate_sim_run_timed(self._h, ms.ctypes.data_as(C.POINTER(C.c_float)), _p(rm, I32P), cap,
+                                        C.byref(n)), "run_timed")
+        return ms[:n.value].astype(np.float64), rm[:n.value].astype(bool)
 
     def synchronize(self):
         _check(lib().kate_sim_synchronize(self._h), "synchronize")

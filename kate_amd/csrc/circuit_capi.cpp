@@ -183,4 +183,14 @@ int kate_sim_create_from_circuit(const kate_circuit* c, const kate_options* opti
     return kate_sim_create(&lp.prog, options, out);
 }
 
+int kate_circuit_plan(const kate_circuit* c, const kate_options* options, int32_t* passes, int32_t* remap_passes,
+                      char* buf, size_t len)
+{
+    if (!c) return KATE_ERR_ARG;
+    Kate::QuantumCircuit qc = c->c;
+    if (qc.instructions.empty()) qc.compileDefault(0, qc.nqbits);
+    Kate::LoweredProgram lp(qc);
+    return kate_plan_describe(&lp.prog, options, passes, remap_passes, buf, len);
+}
+
 }  // extern "C"

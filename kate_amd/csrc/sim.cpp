@@ -163,6 +163,7 @@ struct kate_sim {
     double* d_out = nullptr;
     uint64_t* d_masks = nullptr;
     double* d_factors = nullptr;
+    std::vector<double> h_factors;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     kate_timings last{};
 };
@@ -174,7 +175,9 @@ int upload(T** dst, const std::vector<T>& src, hipStream_t s)
 {
     const size_t bytes = std::max<size_t>(1, src.size()) * sizeof(T);
     KATE_HIP(hipMalloc((void**)dst, bytes));
-    if (!src.empty()) KATE_HIP(hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    // synchronous: the source may be a temporary
+    if (!src.empty()) KATE_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    (void)s;
     return KATE_OK;
 }
 
@@ -224,7 +227,10 @@ int init_product(kate_sim* s, const double* tp)
         f[2 * ((size_t)v + m)] = std::cos(ph) * sn;
         f[2 * ((size_t)v + m) + 1] = std::sin(ph) * sn;
     }
-    KATE_HIP(hipMemcpyAsync(s->d_factors, f.data(), f.size() * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    KATE_HIP(hipStreamSynchronize(s->stream));  // the previous factors may still be in use
+    s->h_factors.swap(f);                        // stays alive until the next product-state init
+    KATE_HIP(hipMemcpyAsync(s->d_factors, s->h_factors.data(), s->h_factors.size() * sizeof(double),
+                            hipMemcpyHostToDevice, s->stream));
     int e = kate::launch_init_product(s->dtype, s->d_state, s->nq_int, s->d_factors, s->stream);
     if (e) return fail(KATE_ERR_HIP, std::string("init launch: ") + hipGetErrorString((hipError_t)e));
     return KATE_OK;
@@ -294,6 +300,20 @@ uint64_t to_virtual(const kate_sim* s, uint64_t real)
 
 }  // namespace
 
+namespace {
+std::string plan_options(const kate_options& opt, kate::PlanOptions& po)
+{
+    if (opt.dtype != KATE_C128 && opt.dtype != KATE_C64) return "dtype must be KATE_C128 or KATE_C64";
+    po.tile_bits = opt.tile_qubits ? opt.tile_qubits : 12;
+    po.low_qubits = opt.low_qubits ? opt.low_qubits : (opt.dtype == KATE_C128 ? 3 : 4);
+    po.min_qubits = po.tile_bits;
+    po.mfma = false;
+    if (po.tile_bits < 10 || po.tile_bits > 13) return "tile_qubits must be in 10..13";
+    if (po.low_qubits < 0 || po.low_qubits > po.tile_bits) return "low_qubits out of range";
+    return "";
+}
+}  // namespace
+
 extern "C" {
 
 const char* kate_status_string(int st)
@@ -348,6 +368,31 @@ int kate_device_info(char* buf, size_t len)
     return KATE_OK;
 }
 
+
+int kate_plan_describe(const kate_program* program, const kate_options* options, int32_t* passes,
+                       int32_t* remap_passes, char* buf, size_t len)
+{
+    kate_options opt{};
+    if (options) opt = *options;
+    kate::PlanOptions po;
+    std::string err = plan_options(opt, po);
+    if (!err.empty()) return fail(KATE_ERR_ARG, err);
+    kate::Program prog;
+    err = to_program(program, prog);
+    if (!err.empty()) return fail(KATE_ERR_PROGRAM, err);
+    kate::Plan plan;
+    err = kate::build_plan(prog, po, plan);
+    if (!err.empty()) return fail(KATE_ERR_PROGRAM, "planning: " + err);
+    if (passes) *passes = plan.gate_passes;
+    if (remap_passes) *remap_passes = plan.remap_passes;
+    if (buf && len) {
+        std::string d = plan.describe();
+        std::strncpy(buf, d.c_str(), len - 1);
+        buf[len - 1] = 0;
+    }
+    return KATE_OK;
+}
+
 int kate_sim_create(const kate_program* program, const kate_options* options, kate_sim** out)
 {
     if (!out) return fail(KATE_ERR_ARG, "null output handle");
@@ -370,17 +415,10 @@ int kate_sim_create(const kate_program* program, const kate_options* options, ka
         return fail(KATE_ERR_PROGRAM, err);
     }
     kate::PlanOptions po;
-    po.tile_bits = opt.tile_qubits ? opt.tile_qubits : 12;
-    po.low_qubits = opt.low_qubits ? opt.low_qubits : (opt.dtype == KATE_C128 ? 3 : 4);
-    po.min_qubits = po.tile_bits;
-    po.mfma = false;
-    if (po.tile_bits < 10 || po.tile_bits > 13) {
+    err = plan_options(opt, po);
+    if (!err.empty()) {
         delete s;
-        return fail(KATE_ERR_ARG, "tile_qubits must be in 10..13");
-    }
-    if (po.low_qubits < 0 || po.low_qubits > po.tile_bits) {
-        delete s;
-        return fail(KATE_ERR_ARG, "low_qubits out of range");
+        return fail(KATE_ERR_ARG, err);
     }
     err = kate::build_plan(s->prog, po, s->plan);
     if (!err.empty()) {
@@ -461,6 +499,37 @@ int kate_sim_run(kate_sim* s)
     if (!s) return fail(KATE_ERR_ARG, "null simulator");
     KATE_HIP(hipSetDevice(s->device));
     return run_passes(s);
+}
+
+int kate_sim_run_timed(kate_sim* s, float* pass_ms, int32_t* pass_is_remap, int32_t cap, int32_t* npasses)
+{
+    if (!s) return fail(KATE_ERR_ARG, "null simulator");
+    KATE_HIP(hipSetDevice(s->device));
+    const size_t np = s->plan.passes.size();
+    if (npasses) *npasses = (int32_t)np;
+    std::vector<hipEvent_t> ev(np + 1, nullptr);
+    int rc = KATE_OK;
+    for (auto& e : ev)
+        if (rc == KATE_OK && hipEventCreate(&e) != hipSuccess) rc = fail(KATE_ERR_HIP, "hipEventCreate");
+    kate::DevPlan dp{s->d_micro, s->d_ops, s->d_coef};
+    if (rc == KATE_OK && hipEventRecord(ev[0], s->stream) != hipSuccess) rc = fail(KATE_ERR_HIP, "hipEventRecord");
+    for (size_t i = 0; i < np && rc == KATE_OK; i++) {
+        int e = kate::launch_pass(s->dtype, s->plan.tile_bits, s->d_state, s->nq_int, s->plan.passes[i], dp, s->stream);
+        if (e) rc = fail(KATE_ERR_HIP, std::string("pass launch: ") + hipGetErrorString((hipError_t)e));
+        else if (hipEventRecord(ev[i + 1], s->stream) != hipSuccess) rc = fail(KATE_ERR_HIP, "hipEventRecord");
+    }
+    if (rc == KATE_OK && hipEventSynchronize(ev[np]) != hipSuccess) rc = fail(KATE_ERR_HIP, "hipEventSynchronize");
+    for (size_t i = 0; i < np && rc == KATE_OK; i++) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) != hipSuccess) rc = fail(KATE_ERR_HIP, "hipEventElapsedTime");
+        if ((int32_t)i < cap) {
+            if (pass_ms) pass_ms[i] = ms;
+            if (pass_is_remap) pass_is_remap[i] = s->plan.pass_is_remap[i];
+        }
+    }
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    return rc;
 }
 
 int kate_sim_measure_proba(kate_sim* s, double* tp)

@@ -1,0 +1,221 @@
+"""GPU parity: libkate_hip.so on the MI355X against the oracle and Kate's own
+outputs.  FP64 (complex128) must agree with Kate's CPU simulator within
+1e-12 max-abs amplitude error; FP32 (complex64) within 1 - fidelity <= 1e-6."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import kate_amd as K
+from fixture_io import unpack_gates
+from kate_amd import workloads as W
+from oracle import oracle as O
+from util import compiled_from_circuit, fidelity, fixtures, load, to_real_order
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12   # max-abs amplitude error, complex128
+FID32 = 1e-6    # 1 - fidelity, complex64
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def gpu_state(c, dtype="c128", proba_in=None, real_order=False, **opt):
+    with K.Simulator(c, dtype=dtype, **opt) as sim:
+        pr = sim.execute(proba_in)
+        return sim.state(real_order=real_order), pr
+
+
+@pytest.mark.parametrize("n,ngates,maxd,seed", [(1, 5, 1, 1), (3, 30, 3, 2), (7, 120, 3, 3), (12, 200, 4, 4),
+                                                (14, 200, 4, 5), (16, 150, 3, 6), (13, 60, 6, 7)])
+def test_random_circuits_match_oracle(gpu, n, ngates, maxd, seed):
+    say("random", n, ngates, maxd, seed)
+    gates = W.random_mixed(n, ngates, seed, max_dense=maxd)
+    c = K.Circuit(n, gates)
+    st, _ = gpu_state(c)
+    ref = O.simulate(n, gates)
+    assert np.max(np.abs(st - ref)) <= TOL64
+
+
+@pytest.mark.parametrize("tile,low", [(10, 2), (11, 4), (12, 3), (13, 5), (12, 0)])
+def test_tile_shapes_match_oracle(gpu, tile, low):
+    say("tile", tile, low)
+    n = 17
+    gates = W.random_mixed(n, 160, 40 + tile, max_dense=3) + W.layered(n, 2, 9)
+    st, _ = gpu_state(K.Circuit(n, gates), tile_qubits=tile, low_qubits=low)
+    assert np.max(np.abs(st - O.simulate(n, gates))) <= TOL64
+
+
+def test_config1_layered20_fp64(gpu):
+    """Config 1: 20 qubits, 10 layers RX/RZ + CNOT ladder, vs Kate's CPU simulator."""
+    say("config1")
+    z, meta = load(fixtures("kate_config1_layered20.npz")[0])
+    gates = unpack_gates(z, "in_")
+    st, pr = gpu_state(K.Circuit(20, gates))
+    err = np.max(np.abs(st[z["sample_index"]] - z["sample_amp"]))
+    say("config1 max-abs vs Kate samples", err)
+    assert err <= TOL64
+    assert np.max(np.abs(st - O.simulate(20, gates))) <= TOL64
+    assert np.max(np.abs(pr - z["proba"])) < 1e-9
+    assert abs(np.vdot(st, st).real - z["norm"]) < 1e-12
+
+
+SIM_FIXTURES = [p for p in fixtures() if "merge" not in p and "config1" not in p]
+
+
+@pytest.mark.parametrize("path", SIM_FIXTURES, ids=lambda p: p.split("/")[-1][5:-4])
+def test_kate_fixtures(gpu, path):
+    z, meta = load(path)
+    if not meta["simulated"]:
+        pytest.skip("Kate's CPU simulator ignores swap instructions: covered by test_swap_programs")
+    say("fixture", path)
+    from test_passes import run_pipeline
+    c = run_pipeline(K.Circuit(meta["n"], unpack_gates(z, "in_")), meta["pipeline"])
+    pin = z["proba_in"] if "proba_in" in z else None
+    st, pr = gpu_state(c, proba_in=pin)
+    assert np.max(np.abs(st - z["state"])) <= TOL64
+    assert np.max(np.abs(pr - z["proba"])) < 1e-9
+
+
+@pytest.mark.parametrize("gpulog2", [1, 3])
+def test_swap_programs(gpu, gpulog2):
+    """Programs compiled for 2^g GPUs run on one: swap instructions become
+    in-HBM remaps; real-order amplitudes equal the swap-free run's."""
+    say("swap", gpulog2)
+    n = 15
+    gates = W.random_mixed(n, 150, 60 + gpulog2)
+    c = K.Circuit(n, gates).schedule().group(6).allocate(gpulog2)
+    assert any(k == 0 for k, _ in c.instructions())
+    prog = compiled_from_circuit(c)
+    st_virtual, _ = gpu_state(c)
+    assert np.max(np.abs(st_virtual - O.run(prog))) <= TOL64
+    st_real, _ = gpu_state(c, real_order=True)
+    plain = K.Circuit(n, c.gates())  # not used for values; swap-free reference below
+    del plain
+    ref_real = to_real_order(O.run(prog), prog.final_inverse_permutation)
+    assert np.max(np.abs(st_real - ref_real)) <= TOL64
+    # the same circuit without allocation, scheduled identically
+    c0 = K.Circuit(n, gates).schedule()
+    st0, _ = gpu_state(c0)
+    assert np.max(np.abs(st_real - st0)) <= 1e-12
+
+
+def test_proba_expect_z_norm(gpu):
+    say("observables")
+    n = 18
+    gates = W.brickwork(n, 12, 5, longrange_every=4, longrange_offset=7)
+    c = K.Circuit(n, gates).schedule().group(7).allocate(0)  # virtual order != real order
+    prog = compiled_from_circuit(c)
+    ref_v = O.run(prog)
+    ref_r = to_real_order(ref_v, prog.final_inverse_permutation)
+    masks = [1 << q for q in range(n)] + [int(m) for m in W.zstring_masks(n, 8, 3)]
+    with K.Simulator(c) as sim:
+        pr = sim.execute()
+        assert np.max(np.abs(pr - O.measure(n, ref_v, prog.final_inverse_permutation))) < 1e-9
+        ez = sim.expect_z(masks)
+        assert np.max(np.abs(ez - O.expect_z(n, ref_r, masks))) < 1e-12
+        assert abs(sim.norm() - 1.0) < 1e-12
+        st = sim.state(real_order=True)
+        assert np.max(np.abs(st - ref_r)) <= TOL64
+
+
+def test_product_state_inputs(gpu):
+    say("product")
+    n = 13
+    rng = np.random.default_rng(3)
+    tp = np.column_stack([rng.uniform(0, 1, n), rng.uniform(0, 6, n)])
+    gates = W.random_mixed(n, 100, 77)
+    c = K.Circuit(n, gates).allocate(0)
+    prog = compiled_from_circuit(c)
+    st, pr = gpu_state(c, proba_in=tp)
+    ref = O.run(prog, tp)
+    assert np.max(np.abs(st - ref)) <= TOL64
+    assert np.max(np.abs(pr - O.measure(n, ref, prog.final_inverse_permutation))) < 1e-9
+
+
+def test_qft_basis_closed_form(gpu):
+    say("qft")
+    n, x = 16, 0b1011001110001011
+    st, _ = gpu_state(K.Circuit(n, W.qft(n)), proba_in=W.basis_input(n, x))
+    rev = int(format(x, f"0{n}b")[::-1], 2)
+    expect = np.exp(2j * np.pi * rev * np.arange(1 << n) / (1 << n)) / 2 ** (n / 2)
+    assert np.max(np.abs(st - expect)) < 1e-12
+
+
+def test_fused_programs_fp64(gpu):
+    say("fused64")
+    n = 16
+    gates = W.layered(n, 6, 4)
+    c = K.Circuit(n, gates).schedule().fuse(5).group(10)
+    st, _ = gpu_state(c)
+    assert np.max(np.abs(st - O.run(compiled_from_circuit(c)))) <= TOL64
+
+
+@pytest.mark.parametrize("n,fuse", [(20, 0), (20, 5), (16, 4)])
+def test_fp32_fidelity(gpu, n, fuse):
+    say("fp32", n, fuse)
+    gates = W.layered(n, 10, 1)
+    c = K.Circuit(n, gates)
+    if fuse:
+        c.schedule().fuse(fuse)
+    st32, _ = gpu_state(c, dtype="c64")
+    ref = O.simulate(n, gates)
+    assert st32.dtype == np.complex64
+    assert 1 - fidelity(st32, ref) <= FID32
+    assert np.max(np.abs(st32 - ref)) < 1e-5
+
+
+def test_mirror_circuit_24q(gpu):
+    """Size-independent property at a larger size: U then U^dagger returns |0>."""
+    say("mirror")
+    n = 24
+    gates = W.brickwork(n, 8, 12, longrange_every=4, longrange_offset=16)
+    inv = [W.Gate(0, g.qubits, 0, g.mat.conj().T.copy()) for g in reversed(gates)]
+    for dtype, tol in (("c128", 1e-12), ("c64", 1e-5)):
+        with K.Simulator(K.Circuit(n, gates + inv), dtype=dtype) as sim:
+            sim.execute()
+            ez = sim.expect_z([1 << q for q in range(n)])
+            assert np.max(np.abs(ez - 1.0)) < 10 * tol
+            st = sim.state(real_order=False)
+            assert abs(st[0] - 1) < tol and np.max(np.abs(st[1:])) < tol
+
+
+def test_repeatable_bitwise(gpu):
+    say("repeat")
+    n = 19
+    gates = W.random_mixed(n, 120, 5)
+    with K.Simulator(K.Circuit(n, gates)) as sim:
+        a = sim.execute()
+        s1 = sim.state()
+        b = sim.execute()
+        s2 = sim.state()
+    assert np.array_equal(a, b) and np.array_equal(s1, s2)
+
+
+def test_drop_in_binary(gpu, tmp_path):
+    """The unchanged Kate user program, linked against the library, agrees with the oracle."""
+    say("drop-in")
+    exe = tmp_path / "drop_in"
+    cmd = ["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"), "-I",
+           os.path.join(ROOT, "kate_amd", "kate"), os.path.join(ROOT, "tests", "cpp", "drop_in_main.cpp"), "-o",
+           str(exe), "-L", os.path.join(ROOT, "kate_amd"), "-lkate_hip", "-Wl,-rpath," + os.path.join(ROOT, "kate_amd")]
+    subprocess.run(cmd, check=True)
+    n = 11
+    out = subprocess.run([str(exe), str(n)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout[-1000:] + out.stderr[-1000:]
+    amps = np.zeros(1 << n, dtype=np.complex128)
+    for line in out.stdout.splitlines():
+        if line.startswith("A "):
+            _, i, re_, im_ = line.split()
+            amps[int(i)] = float(re_) + 1j * float(im_)
+    gates = W.qft(n) + [W.Gate(W.H, (0,)), W.Gate(W.CNOT, (1, 0)),
+                        W.Gate(0, (n - 1,), 0, np.array([[0, 1], [1, 0]], dtype=np.complex128))]
+    tp = np.array([[float(i % 2), 0.0] for i in range(n)])
+    ref = O.simulate(n, gates, tp)
+    assert np.max(np.abs(amps - ref)) < 1e-12

@@ -1,0 +1,165 @@
+"""Pins the oracle (oracle/kate_oracle.c): against Kate's own outputs (golden
+fixtures, and Kate's CPU build live when it is present), and against what
+the mathematics fixes (closed forms, brute-force unitaries, invariants)."""
+import math
+
+import numpy as np
+import pytest
+
+from fixture_io import unpack_gates
+from kate_amd import workloads as W
+from oracle import oracle as O
+from util import compiled_from_fixture, dense_unitary, fixtures, load, to_real_order
+
+SIM_FIXTURES = [p for p in fixtures() if "merge" not in p and "config1" not in p]
+
+
+@pytest.mark.parametrize("path", SIM_FIXTURES, ids=lambda p: p.split("/")[-1][5:-4])
+def test_oracle_equals_kate_fixture(path):
+    z, meta = load(path)
+    if not meta["simulated"]:
+        pytest.skip("program has swap instructions; Kate's CPU simulator ignores them")
+    prog = compiled_from_fixture(z, meta)
+    pin = z["proba_in"] if "proba_in" in z else None
+    st = O.run(prog, pin)
+    assert np.array_equal(st, z["state"]), np.max(np.abs(st - z["state"]))
+    pr = O.measure(meta["n"], st, prog.final_inverse_permutation)
+    assert np.array_equal(pr, z["proba"])
+
+
+def test_oracle_equals_kate_config1_layered20():
+    z, meta = load(fixtures("kate_config1_layered20.npz")[0])
+    gates = unpack_gates(z, "in_")
+    st = O.simulate(20, gates)
+    assert np.array_equal(st[z["sample_index"]], z["sample_amp"])
+    assert np.array_equal(O.measure(20, st), z["proba"])
+    assert st.sum() == z["sum"]
+
+
+@pytest.mark.parametrize("path", fixtures("kate_merge_*.npz"), ids=lambda p: p.split("/")[-1][5:-4])
+def test_oracle_merge_equals_kate(path):
+    z, _ = load(path)
+    qs, m = O.merge(unpack_gates(z, "in_"))
+    assert qs == tuple(int(x) for x in z["qubits"])
+    assert np.array_equal(m, z["mat"])
+
+
+def test_merge_is_the_product_of_the_members():
+    gates = W.random_mixed(5, 12, 9, max_dense=3)
+    qs, m = O.merge(gates)
+    n = len(qs)
+    remap = {q: i for i, q in enumerate(qs)}
+    local = [W.Gate(g.id, tuple(remap[q] for q in g.qubits), g.optarg, g.mat) for g in gates]
+    assert np.allclose(m, dense_unitary(n, local), atol=1e-13)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_oracle_matches_brute_force_unitary(seed):
+    n = 6
+    gates = W.random_mixed(n, 40, 100 + seed, max_dense=3)
+    st = O.simulate(n, gates)
+    U = dense_unitary(n, gates)
+    assert np.max(np.abs(st - U[:, 0])) < 1e-13
+
+
+def test_hadamard_layer_closed_form():
+    n = 7
+    st = O.simulate(n, [W.Gate(W.H, (q,)) for q in range(n)])
+    assert np.max(np.abs(st - 2 ** (-n / 2))) < 1e-15
+
+
+def test_qft_on_basis_input_closed_form():
+    n, x = 9, 0b101101011
+    st = O.simulate(n, W.qft(n), W.basis_input(n, x))
+    # Kate's QFT (H on qubit i, then phases 2 pi / 2^(j-i+1) from every j > i,
+    # no final swaps) leaves qubit i with phase 2 pi 0.x_i x_{i+1}..x_{n-1}, so
+    # |x> -> 2^{-n/2} sum_z exp(2 pi i rev(x) z / 2^n) |z>, rev = n-bit reversal.
+    rev = int(format(x, f"0{n}b")[::-1], 2)
+    z = np.arange(1 << n)
+    expect = np.exp(2j * np.pi * rev * z / (1 << n)) / 2 ** (n / 2)
+    assert np.max(np.abs(st - expect)) < 1e-13
+
+
+def test_product_state_init_and_proba_closed_form():
+    n = 6
+    rng = np.random.default_rng(4)
+    tp = np.column_stack([rng.uniform(0.05, 0.95, n), rng.uniform(-1.4, 1.4, n)])
+    st = O.simulate(n, [], tp)
+    f0 = np.cos(tp[:, 0] * np.pi / 2)
+    f1 = np.exp(1j * tp[:, 1]) * np.sin(tp[:, 0] * np.pi / 2)
+    expect = np.ones(1 << n, dtype=np.complex128)
+    for q in range(n):
+        bit = (np.arange(1 << n) >> q) & 1
+        expect *= np.where(bit == 1, f1[q], f0[q])
+    assert np.max(np.abs(st - expect)) < 1e-15
+    # Kate's proba_state of a product state gives back (theta, phi) (phi mod pi)
+    pr = O.measure(n, st)
+    assert np.max(np.abs(pr[:, 0] - tp[:, 0])) < 1e-12
+    assert np.max(np.abs(pr[:, 1] - tp[:, 1])) < 1e-12
+    # <Z_q> = cos(theta pi); Z strings multiply
+    zq = O.expect_z(n, st, [1 << q for q in range(n)] + [0b101001])
+    assert np.max(np.abs(zq[:n] - np.cos(tp[:, 0] * np.pi))) < 1e-13
+    assert abs(zq[n] - np.prod(np.cos(tp[[0, 3, 5], 0] * np.pi))) < 1e-13
+
+
+def test_basis_state_proba_thresholds():
+    n = 5
+    st = O.simulate(n, [], W.basis_input(n, 0b10110))
+    pr = O.measure(n, st)
+    assert pr[:, 0].tolist() == [0, 1, 1, 0, 1] and pr[:, 1].tolist() == [0] * 5
+
+
+def test_swap_bits_is_a_qubit_transposition():
+    n = 6
+    st = np.random.default_rng(1).standard_normal(1 << n) + 0j
+    out = O.swap_bits(n, st, 1, 4)
+    t = st.reshape([2] * n)  # axis k <-> bit n-1-k
+    expect = np.swapaxes(t, n - 1 - 1, n - 1 - 4).ravel()
+    assert np.array_equal(out, expect)
+
+
+def test_swap_program_equals_swap_free_program():
+    """A program compiled for 4 GPUs (swap instructions) gives the same real-
+    order amplitudes as the same circuit compiled for one."""
+    pytest.importorskip("kate_amd")
+    import kate_amd as K
+    from util import compiled_from_circuit
+    n = 9
+    gates = W.random_mixed(n, 70, 21)
+    a = K.Circuit(n, gates).group(4).compile_default(2)
+    assert any(k == 0 for k, _ in a.instructions())
+    pa = compiled_from_circuit(a)
+    sa = to_real_order(O.run(pa), pa.final_inverse_permutation)
+    sb = O.simulate(n, gates)
+    assert np.array_equal(sa, sb)
+
+
+def test_mirror_circuit_returns_to_zero():
+    n = 8
+    gates = W.brickwork(n, 10, 3, longrange_every=3, longrange_offset=3)
+    inv = [W.Gate(0, g.qubits, 0, g.mat.conj().T) for g in reversed(gates)]
+    st = O.simulate(n, gates + inv)
+    e = np.zeros(1 << n, dtype=np.complex128)
+    e[0] = 1
+    assert np.max(np.abs(st - e)) < 1e-13
+
+
+def test_planted_mistake_is_caught():
+    """The fixture comparison detects a single wrong gate in the oracle's input."""
+    z, meta = load(fixtures("kate_mixed8_raw.npz")[0])
+    prog = compiled_from_fixture(z, meta)
+    g = prog.gates[17]
+    prog.gates[17] = W.Gate(g.id, tuple(reversed(g.qubits)), g.optarg,
+                            None if g.mat is None else g.mat.T.copy())
+    assert not np.array_equal(O.run(prog), z["state"])
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="Kate's CPU build (oracle/_ref) is not built here")
+@pytest.mark.parametrize("seed", range(3))
+def test_oracle_equals_live_kate(seed):
+    n = 10
+    gates = W.random_mixed(n, 120, 300 + seed, max_dense=4)
+    st_ref, pr_ref = O.KateRef(n, gates).simulate()
+    st = O.simulate(n, gates)
+    assert np.array_equal(st, st_ref)
+    assert np.array_equal(O.measure(n, st), pr_ref)

@@ -1,0 +1,124 @@
+"""The product's compiler passes (kate_amd/kate/kate_circuit.hpp, reached
+through the C ABI) against Kate's own passes: golden fixtures, and Kate's CPU
+build live where it is present.  Bit-exact: same gate order, same fused
+matrices, same groups, instructions and permutations."""
+import numpy as np
+import pytest
+
+import kate_amd as K
+from fixture_io import unpack_gates, unpack_program
+from kate_amd import workloads as W
+from oracle import oracle as O
+from util import fixtures, load
+
+PASS_FIXTURES = [p for p in fixtures() if "merge" not in p and "config1" not in p]
+
+
+def run_pipeline(c, steps):
+    for s in steps:
+        if s == "schedule":
+            c.schedule()
+        elif s.startswith("fuse"):
+            c.fuse(int(s[4:]), 1e-6)
+        elif s.startswith("group"):
+            c.group(int(s[5:]))
+        elif s.startswith("allocate"):
+            c.allocate(int(s[8:]))
+        elif s.startswith("cd"):
+            c.compile_default(int(s[2:]))
+    return c
+
+
+def assert_same_gates(ours, theirs):
+    assert len(ours) == len(theirs)
+    for i, (a, b) in enumerate(zip(ours, theirs)):
+        assert (a.id, tuple(a.qubits)) == (b.id, tuple(b.qubits)), i
+        if a.id == 0:
+            assert np.array_equal(a.mat, b.mat), i
+        else:
+            assert a.optarg == b.optarg, i
+
+
+@pytest.mark.parametrize("path", PASS_FIXTURES, ids=lambda p: p.split("/")[-1][5:-4])
+def test_passes_equal_kate_fixture(path):
+    z, meta = load(path)
+    c = run_pipeline(K.Circuit(meta["n"], unpack_gates(z, "in_")), meta["pipeline"])
+    assert_same_gates(c.gates(), unpack_gates(z, "out_"))
+    if meta.get("compiled") or meta.get("grouped"):
+        groups, ins, perms = unpack_program(z, "prog_")
+        assert c.groups() == groups
+        if meta.get("compiled"):
+            assert c.instructions() == ins
+            assert c.permutations() == perms
+
+
+def test_fusion_composes_in_program_order():
+    n = 5
+    gates = W.random_mixed(n, 10, 2, max_dense=2)
+    c = K.Circuit(n, gates).fuse(n, 0.0)
+    fused = c.gates()
+    # the fused program applied by the oracle equals the unfused one to rounding
+    a = O.simulate(n, gates)
+    b = O.simulate(n, fused)
+    assert np.max(np.abs(a - b)) < 1e-13
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="Kate's CPU build (oracle/_ref) is not built here")
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("steps", [["schedule", "fuse5", "group7"], ["schedule", "group6", "allocate2"],
+                                   ["fuse4", "group5", "cd1"]], ids=["sfg", "sga", "fgc"])
+def test_passes_equal_live_kate(seed, steps):
+    n = 11
+    gates = W.random_mixed(n, 90, 500 + seed, max_dense=3) + W.layered(n, 2, seed)
+    c = run_pipeline(K.Circuit(n, gates), steps)
+    ref = O.KateRef(n, gates)
+    for s in steps:
+        if s == "schedule":
+            ref.schedule()
+        elif s.startswith("fuse"):
+            ref.fuse(int(s[4:]), 1e-6)
+        elif s.startswith("group"):
+            ref.group(int(s[5:]))
+        elif s.startswith("allocate"):
+            ref.allocate(int(s[8:]))
+        elif s.startswith("cd"):
+            ref.compile_default(int(s[2:]))
+    assert_same_gates(c.gates(), [W.Gate(i, q, o, m) for (i, q, o, m, _t) in ref.gates()])
+    assert c.groups() == ref.groups()
+    assert c.instructions() == ref.instructions()
+    if ref.instructions():
+        assert c.permutations() == ref.perms()
+
+
+def test_schedule_tie_break_follows_reference_arithmetic():
+    """Kate's scheduling weight is computed in size_t and wraps: a candidate
+    whose qubits do not contain the previous gate's qubits is preferred over a
+    later candidate that does.  This small case separates that rule from the
+    'maximum overlap' reading of the code."""
+    n = 4
+    gates = [W.Gate(W.H, (0,)), W.Gate(W.H, (1,)), W.Gate(W.CNOT, (1, 0)), W.Gate(W.H, (2,)), W.Gate(W.H, (3,)),
+             W.Gate(W.CNOT, (3, 2))]
+    c = K.Circuit(n, gates).schedule()
+    order = [(g.id, g.qubits) for g in c.gates()]
+    if O.ref_available():
+        ref = O.KateRef(n, gates)
+        ref.schedule()
+        assert order == [(g[0], g[1]) for g in ref.gates()]
+    assert sorted(order) == sorted((g.id, g.qubits) for g in gates)
+
+
+def test_plan_covers_layered_30q_in_few_passes():
+    n = 30
+    c = K.Circuit(n, W.layered(n, 20, 1))
+    passes, remaps, text = c.plan("c128")
+    assert remaps == 0
+    assert 5 <= passes <= 40
+    assert f"ops={20 * (2 * n + n - 1)}" in text.splitlines()[0]
+
+
+def test_plan_remaps_swap_instructions():
+    n = 16
+    c = K.Circuit(n, W.random_mixed(n, 100, 8)).group(6).compile_default(3)
+    nswap = sum(1 for k, _ in c.instructions() if k == 0)
+    passes, remaps, _ = c.plan("c128")
+    assert nswap > 0 and remaps >= nswap
