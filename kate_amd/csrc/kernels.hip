@@ -147,6 +147,18 @@ __device__ __forceinline__ void op_toff(Cx<R> (&v)[16])
     }
 }
 
+// Independent 2x2 gates on every micro bit of MASK (one matrix per bit, in
+// ascending bit order), applied one after the other.
+template <int MASK, typename R>
+__device__ __forceinline__ void op_kron(Cx<R> (&v)[16], const Cx<R>* __restrict__ m)
+{
+    int k = 0;
+    if (MASK & 1) op_dense<1>(v, m + 4 * k++);
+    if (MASK & 2) op_dense<2>(v, m + 4 * k++);
+    if (MASK & 4) op_dense<4>(v, m + 4 * k++);
+    if (MASK & 8) op_dense<8>(v, m + 4 * k++);
+}
+
 #define KATE_OPC(kind, var) (((kind) << 8) | (var))
 
 template <typename R>
@@ -154,7 +166,8 @@ __device__ __forceinline__ void apply_op(Cx<R> (&v)[16], uint32_t code, const Cx
 {
     switch (code) {
 #define D(M) \
-    case KATE_OPC(OK_DENSE, M): op_dense<M>(v, cf); break;
+    case KATE_OPC(OK_DENSE, M): op_dense<M>(v, cf); break; \
+    case KATE_OPC(OK_KRON, M): op_kron<M>(v, cf); break;
         D(1) D(2) D(3) D(4) D(5) D(6) D(7) D(8) D(9) D(10) D(11) D(12) D(13) D(14) D(15)
 #undef D
 #define P(J) \
@@ -180,23 +193,24 @@ __device__ __forceinline__ void apply_op(Cx<R> (&v)[16], uint32_t code, const Cx
     }
 }
 
-// Index of the `line`-th 16-amplitude group: zero bits inserted at the four
-// ascending tile bits p0 < p1 < p2 < p3.
-__device__ __forceinline__ uint32_t spread4(uint32_t line, uint32_t p0, uint32_t p1, uint32_t p2, uint32_t p3)
+// LDS index of tile index y under a layout (see MicroStep::cols).
+template <int TB>
+__device__ __forceinline__ uint32_t layout(uint32_t y, const uint16_t* __restrict__ cols)
 {
-    uint32_t lo;
-    lo = line & ((1u << p0) - 1u); line = lo | ((line - lo) << 1);
-    lo = line & ((1u << p1) - 1u); line = lo | ((line - lo) << 1);
-    lo = line & ((1u << p2) - 1u); line = lo | ((line - lo) << 1);
-    lo = line & ((1u << p3) - 1u); line = lo | ((line - lo) << 1);
-    return line;
+    uint32_t p = 0;
+#pragma unroll
+    for (int b = 0; b < TB; b++)
+        if ((y >> b) & 1u) p ^= cols[b];
+    return p;
 }
 
 // One dense gate of any width on the LDS tile: every thread computes its 16
 // outputs from LDS, then (after a barrier) writes them.
 template <typename R, int TB>
-__device__ __forceinline__ void generic_dense(Cx<R>* tile, const MicroStep& ms, const Cx<R>* __restrict__ m)
+__device__ __forceinline__ void generic_dense(Cx<R>* tile, const MicroStep* __restrict__ msp,
+                                              const Cx<R>* __restrict__ m)
 {
+    const MicroStep& ms = *msp;
     constexpr int NT = 1 << (TB - 4);
     const int k = ms.nbits;
     const uint32_t dim = 1u << k;
@@ -208,18 +222,19 @@ __device__ __forceinline__ void generic_dense(Cx<R>* tile, const MicroStep& ms, 
         const uint32_t j = (uint32_t)e * NT + threadIdx.x;
         uint32_t row = 0;
         for (int i = 0; i < k; i++) row |= ((j >> ms.bits[i]) & 1u) << i;
-        const uint32_t base = j & ~gmask;
+        const uint32_t base = layout<TB>(j & ~gmask, ms.cols);
         Cx<R> acc{0, 0};
         for (uint32_t c = 0; c < dim; c++) {
             uint32_t idx = base;
-            for (int i = 0; i < k; i++) idx |= ((c >> i) & 1u) << ms.bits[i];
+            for (int i = 0; i < k; i++)
+                if ((c >> i) & 1u) idx ^= ms.cols[ms.bits[i]];
             acc = cmac(acc, m[row * dim + c], tile[swz<R>(idx)]);
         }
         out[e] = acc;
     }
     __syncthreads();
 #pragma unroll
-    for (int e = 0; e < 16; e++) tile[swz<R>((uint32_t)e * NT + threadIdx.x)] = out[e];
+    for (int e = 0; e < 16; e++) tile[swz<R>(layout<TB>((uint32_t)e * NT + threadIdx.x, ms.cols))] = out[e];
 }
 
 // WIDE instantiations also run SK_GENERIC micro-steps; keeping them out of
@@ -257,37 +272,47 @@ __global__ __launch_bounds__(1 << (TB - 4)) void k_pass(Cx<R>* __restrict__ stat
     __syncthreads();
 
     for (uint32_t mi = 0; mi < pa.micro_count; mi++) {
-        const MicroStep ms = micro[pa.micro_begin + mi];
-        if (ms.kind == SK_REG) {
-            const uint32_t o0 = 1u << ms.bits[0], o1 = 1u << ms.bits[1], o2 = 1u << ms.bits[2],
-                           o3 = 1u << ms.bits[3];
-            const uint32_t b = spread4(tid, ms.bits[0], ms.bits[1], ms.bits[2], ms.bits[3]);
+        const MicroStep* __restrict__ msp = micro + pa.micro_begin + mi;
+        if (msp->kind == SK_REG) {
+            const uint32_t o0 = msp->pcols[0], o1 = msp->pcols[1], o2 = msp->pcols[2], o3 = msp->pcols[3];
+            uint32_t b = 0;
+#pragma unroll
+            for (int i = 0; i < TB - 4; i++)
+                if ((tid >> i) & 1u) b ^= msp->tcols[i];
+            const uint32_t op_begin = msp->op_begin, op_count = msp->op_count;
             Cx<R> v[16];
 #pragma unroll
             for (int s = 0; s < 16; s++)
-                v[s] = tile[swz<R>(b | ((s & 1) ? o0 : 0) | ((s & 2) ? o1 : 0) | ((s & 4) ? o2 : 0) | ((s & 8) ? o3 : 0))];
-            for (uint32_t o = 0; o < ms.op_count; o++) {
-                const MicroOp op = ops[ms.op_begin + o];
+                v[s] = tile[swz<R>(b ^ ((s & 1) ? o0 : 0) ^ ((s & 2) ? o1 : 0) ^ ((s & 4) ? o2 : 0) ^ ((s & 8) ? o3 : 0))];
+            for (uint32_t o = 0; o < op_count; o++) {
+                const MicroOp op = ops[op_begin + o];
                 apply_op<R>(v, op.code, coef + op.coef);
             }
 #pragma unroll
             for (int s = 0; s < 16; s++)
-                tile[swz<R>(b | ((s & 1) ? o0 : 0) | ((s & 2) ? o1 : 0) | ((s & 4) ? o2 : 0) | ((s & 8) ? o3 : 0))] = v[s];
+                tile[swz<R>(b ^ ((s & 1) ? o0 : 0) ^ ((s & 2) ? o1 : 0) ^ ((s & 4) ? o2 : 0) ^ ((s & 8) ? o3 : 0))] = v[s];
         } else if (WIDE) {
-            generic_dense<R, TB>(tile, ms, coef + ms.coef);
+            generic_dense<R, TB>(tile, msp, coef + msp->coef);
         }
         __syncthreads();
     }
 
-    {   // store
+    {   // store, reading the tile through its final layout
         uint64_t tb = base;
-        for (int b = 0; b < TB - 4; b++) tb |= (uint64_t)((tid >> b) & 1u) << pa.store_pos[b];
+        uint32_t lt = 0;
+        for (int b = 0; b < TB - 4; b++) {
+            tb |= (uint64_t)((tid >> b) & 1u) << pa.store_pos[b];
+            if ((tid >> b) & 1u) lt ^= pa.store_cols[b];
+        }
         const uint64_t m0 = 1ull << pa.store_pos[TB - 4], m1 = 1ull << pa.store_pos[TB - 3];
         const uint64_t m2 = 1ull << pa.store_pos[TB - 2], m3 = 1ull << pa.store_pos[TB - 1];
+        const uint32_t l0 = pa.store_cols[TB - 4], l1 = pa.store_cols[TB - 3], l2 = pa.store_cols[TB - 2],
+                       l3 = pa.store_cols[TB - 1];
 #pragma unroll
         for (int i = 0; i < 16; i++) {
             const uint64_t off = tb + ((i & 1) ? m0 : 0) + ((i & 2) ? m1 : 0) + ((i & 4) ? m2 : 0) + ((i & 8) ? m3 : 0);
-            state[off] = tile[swz<R>((uint32_t)i * NT + tid)];
+            const uint32_t li = lt ^ ((i & 1) ? l0 : 0) ^ ((i & 2) ? l1 : 0) ^ ((i & 4) ? l2 : 0) ^ ((i & 8) ? l3 : 0);
+            state[off] = tile[swz<R>(li)];
         }
     }
 }

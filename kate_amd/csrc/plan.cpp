@@ -47,6 +47,65 @@ int sweep(const std::vector<uint64_t>& masks, const std::vector<uint8_t>& ok, ui
     return count;
 }
 
+// The sweep of register micro-steps.  CNOTs (lazy[i]) never need their
+// qubits in registers: one whose qubits are held and not pending runs as a
+// register op, any other is applied to the tile layout after the step; a
+// gate after a layout CNOT on one of its qubits waits for the next step.
+int sweep_micro(const std::vector<uint64_t>& masks, const std::vector<uint8_t>& ok, const std::vector<uint8_t>& lazy,
+                uint64_t set, std::vector<int>* frontier, std::vector<int>* reg, std::vector<int>* late)
+{
+    uint64_t blocked = 0, pending = 0;
+    int count = 0;
+    for (size_t i = 0; i < masks.size(); i++) {
+        const uint64_t m = masks[i];
+        if (m & blocked) {
+            blocked |= m;
+        } else if (lazy[i]) {
+            count++;
+            if ((m & ~set) == 0 && !(m & pending)) {
+                if (reg) reg->push_back((int)i);
+            } else {
+                pending |= m;
+                if (late) late->push_back((int)i);
+            }
+        } else if (ok[i] && (m & ~set) == 0 && !(m & pending)) {
+            count++;
+            if (reg) reg->push_back((int)i);
+        } else {
+            if (frontier && ok[i] && !(m & pending)) frontier->push_back((int)i);
+            blocked |= m;
+        }
+    }
+    return count;
+}
+
+uint64_t choose_micro_set(const std::vector<uint64_t>& masks, const std::vector<uint8_t>& ok,
+                          const std::vector<uint8_t>& lazy)
+{
+    uint64_t set = 0;
+    for (;;) {
+        std::vector<int> frontier;
+        const int now = sweep_micro(masks, ok, lazy, set, &frontier, nullptr, nullptr);
+        double best = 0;
+        uint64_t best_new = 0;
+        std::vector<uint64_t> tried;
+        for (int f : frontier) {
+            const uint64_t add = masks[(size_t)f] & ~set;
+            if (popc(set | add) > 4) continue;
+            if (std::find(tried.begin(), tried.end(), add) != tried.end()) continue;
+            tried.push_back(add);
+            const int gain = sweep_micro(masks, ok, lazy, set | add, nullptr, nullptr, nullptr) - now;
+            const double score = (double)gain / popc(add);
+            if (score > best) {
+                best = score;
+                best_new = add;
+            }
+        }
+        if (!best_new) return set;
+        set |= best_new;
+    }
+}
+
 // Greedy growth of a qubit set up to `capacity`: repeatedly add the new
 // qubits of the ready gate that makes the most gates runnable per added
 // qubit (earliest gate on ties).
@@ -166,28 +225,135 @@ struct Builder {
         return "";
     }
 
+    // A run of 1-qubit gates on one micro bit, composed in FP64.
+    struct Held {
+        int count = 0;
+        bool single_h = false;
+        double m[8] = {0};  // 2x2 complex, row-major
+    };
+
+    static int micro_bit(int tile_bit, const uint8_t* bits)
+    {
+        for (int j = 0; j < 4; j++)
+            if (bits[j] == tile_bit) return j;
+        return -1;
+    }
+
+    void hold(Held& h, const PlanGate& g)
+    {
+        double gm[8];
+        if (g.id == 2) {
+            const double s = 0.7071067811865475;
+            const double hm[8] = {s, 0, s, 0, s, 0, -s, 0};
+            std::copy(hm, hm + 8, gm);
+        } else {
+            std::copy(prog.mat.data() + 2 * g.moff, prog.mat.data() + 2 * g.moff + 8, gm);
+        }
+        if (h.count == 0) {
+            std::copy(gm, gm + 8, h.m);
+            h.single_h = g.id == 2;
+        } else {
+            double out[8];
+            for (int r = 0; r < 2; r++)
+                for (int c = 0; c < 2; c++) {
+                    double re = 0, im = 0;
+                    for (int k = 0; k < 2; k++) {
+                        const double ar = gm[2 * (r * 2 + k)], ai = gm[2 * (r * 2 + k) + 1];
+                        const double br = h.m[2 * (k * 2 + c)], bi = h.m[2 * (k * 2 + c) + 1];
+                        re += ar * br - ai * bi;
+                        im += ar * bi + ai * br;
+                    }
+                    out[2 * (r * 2 + c)] = re;
+                    out[2 * (r * 2 + c) + 1] = im;
+                }
+            std::copy(out, out + 8, h.m);
+            h.single_h = false;
+        }
+        h.count++;
+    }
+
+    // Emits the held gates of the micro bits in `mask`: a lone Hadamard keeps
+    // Kate's Hadamard arithmetic, a diagonal becomes a phase op, the rest go
+    // out together as one Kronecker op.
+    void flush(Held* held, int mask)
+    {
+        int kron = 0;
+        for (int b = 0; b < 4; b++) {
+            if (!((mask >> b) & 1) || held[b].count == 0) continue;
+            Held& h = held[b];
+            MicroOp op{};
+            if (h.count == 1 && h.single_h) {
+                op.code = (uint16_t)(OK_H << 8 | b);
+                plan.ops.push_back(op);
+                h = Held();
+            } else if (h.m[2] == 0.0 && h.m[3] == 0.0 && h.m[4] == 0.0 && h.m[5] == 0.0) {
+                const double d[4] = {h.m[0], h.m[1], h.m[6], h.m[7]};
+                op.coef = add_coef(d, 2);
+                op.code = (uint16_t)(OK_PHASE << 8 | b);
+                plan.ops.push_back(op);
+                h = Held();
+            } else {
+                kron |= 1 << b;
+            }
+        }
+        if (!kron) return;
+        MicroOp op{};
+        op.coef = (uint32_t)(plan.coef.size() / 2);
+        for (int b = 0; b < 4; b++)
+            if ((kron >> b) & 1) {
+                add_coef(held[b].m, 4);
+                held[b] = Held();
+            }
+        op.code = (uint16_t)((__builtin_popcount(kron) == 1 ? OK_DENSE : OK_KRON) << 8 | kron);
+        plan.ops.push_back(op);
+    }
+
     // Packs the gates of one pass (`gl`, program order) into micro-steps.
     std::string pack_micro(const std::vector<int>& gl, const int* tile_of, PassArgs& pa)
     {
         const int t = plan.tile_bits;
         std::vector<int> rem = gl;
         pa.micro_begin = (uint32_t)plan.micro.size();
+        uint16_t cols[16];
+        for (int b = 0; b < 16; b++) cols[b] = (uint16_t)(1u << b);
+        // A CNOT on tile bits (target tb, control cb) maps tile index y to
+        // y ^ (y_cb << tb); the amplitude of index y then sits where y's image
+        // sat, so the layout column of the control absorbs the target's.
+        auto layout_cnot = [&](const PlanGate& g) {
+            const int tb = tile_of[g.q[0]], cb = tile_of[g.q[1]];
+            cols[cb] = (uint16_t)(cols[cb] ^ cols[tb]);
+        };
         while (!rem.empty()) {
             std::vector<uint64_t> tm(rem.size());
-            std::vector<uint8_t> ok(rem.size());
-            uint64_t universe = 0;
+            std::vector<uint8_t> ok(rem.size()), lazy(rem.size());
             for (size_t i = 0; i < rem.size(); i++) {
                 const PlanGate& g = prog.gates[(size_t)rem[i]];
                 uint64_t m = 0;
                 for (int j = 0; j < g.k; j++) m |= 1ull << tile_of[g.q[j]];
                 tm[i] = m;
                 ok[i] = g.k <= 4 && !(g.id == 0 && opt.mfma && g.k == 4);
-                universe |= m;
+                lazy[i] = g.id == 3;
             }
-            const uint64_t set = choose_set(tm, ok, 4, 0, universe);
-            std::vector<int> acc;
-            sweep(tm, ok, set, universe, nullptr, &acc);
+            const uint64_t set = choose_micro_set(tm, ok, lazy);
+            std::vector<int> acc, late;
+            sweep_micro(tm, ok, lazy, set, nullptr, &acc, &late);
             MicroStep ms{};
+            std::copy(cols, cols + 16, ms.cols);
+            if (acc.empty() && !late.empty()) {
+                // only layout CNOTs are ready: apply them, no data moves
+                for (int a : late) layout_cnot(prog.gates[(size_t)rem[(size_t)a]]);
+                std::vector<int> next;
+                size_t li = 0;
+                for (size_t i = 0; i < rem.size(); i++) {
+                    if (li < late.size() && late[li] == (int)i) {
+                        li++;
+                        continue;
+                    }
+                    next.push_back(rem[i]);
+                }
+                rem.swap(next);
+                continue;
+            }
             if (acc.empty()) {
                 // the first remaining gate is one the register path does not take
                 const PlanGate& g = prog.gates[(size_t)rem[0]];
@@ -207,16 +373,45 @@ struct Builder {
             for (int b = 0; b < t; b++)
                 if ((full >> b) & 1ull) ms.bits[nb++] = (uint8_t)b;
             ms.op_begin = (uint32_t)plan.ops.size();
-            for (int a : acc) {
-                std::string e = add_reg_op(prog.gates[(size_t)rem[(size_t)a]], tile_of, ms.bits);
-                if (!e.empty()) return e;
+            {
+                // 1-qubit gates are held per micro bit and multiplied together
+                // (FP64, later gate on the left) until a gate on more qubits
+                // touches the bit; held gates of several bits go out as one op.
+                Held held[4];
+                for (int a : acc) {
+                    const PlanGate& g = prog.gates[(size_t)rem[(size_t)a]];
+                    int bits[16];
+                    for (int i = 0; i < g.k; i++) bits[i] = micro_bit(tile_of[g.q[i]], ms.bits);
+                    if (g.k == 1) {
+                        hold(held[bits[0]], g);
+                        continue;
+                    }
+                    int touched = 0;
+                    for (int i = 0; i < g.k; i++) touched |= 1 << bits[i];
+                    flush(held, touched);
+                    std::string e = add_reg_op(g, tile_of, ms.bits);
+                    if (!e.empty()) return e;
+                }
+                flush(held, 15);
             }
             ms.op_count = (uint32_t)(plan.ops.size() - ms.op_begin);
+            {
+                int ti = 0;
+                for (int b = 0; b < t; b++) {
+                    const bool reg = b == ms.bits[0] || b == ms.bits[1] || b == ms.bits[2] || b == ms.bits[3];
+                    if (!reg && ti < 12) ms.tcols[ti++] = ms.cols[b];
+                }
+                for (int k = 0; k < 4; k++) ms.pcols[k] = ms.cols[ms.bits[k]];
+            }
             plan.micro.push_back(ms);
+            for (int a : late) layout_cnot(prog.gates[(size_t)rem[(size_t)a]]);
+            std::vector<int> done(acc);
+            done.insert(done.end(), late.begin(), late.end());
+            std::sort(done.begin(), done.end());
             std::vector<int> next;
             size_t ai = 0;
             for (size_t i = 0; i < rem.size(); i++) {
-                if (ai < acc.size() && acc[ai] == (int)i) {
+                if (ai < done.size() && done[ai] == (int)i) {
                     ai++;
                     continue;
                 }
@@ -224,6 +419,7 @@ struct Builder {
             }
             rem.swap(next);
         }
+        std::copy(cols, cols + 16, pa.store_cols);
         pa.micro_count = (uint32_t)(plan.micro.size() - pa.micro_begin);
         pa.has_wide = 0;
         for (uint32_t m = pa.micro_begin; m < plan.micro.size(); m++)
@@ -323,6 +519,7 @@ struct Builder {
             for (int b = 0; b < pa.tile_bits; b++) pa.store_pos[b] = (uint8_t)where[pa.load_pos[b]];
             pa.micro_begin = (uint32_t)plan.micro.size();
             pa.micro_count = 0;
+            for (int b = 0; b < 16; b++) pa.store_cols[b] = (uint16_t)(1u << b);
             plan.passes.push_back(pa);
             plan.pass_is_remap.push_back(1);
             plan.remap_passes++;

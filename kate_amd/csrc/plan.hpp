@@ -17,6 +17,7 @@ enum OpKind : uint16_t {
     OK_CNOT = 3,   // variant = target*4 + control
     OK_CRK = 4,    // variant = a*4 + b, a < b
     OK_TOFF = 5,   // variant = target*16 + c0*4 + c1, c0 < c1
+    OK_KRON = 6,   // variant = mask of micro bits; one 2x2 per set bit, ascending, applied in turn
 };
 
 enum StepKind : uint8_t {
@@ -39,6 +40,15 @@ struct MicroStep {
     uint32_t op_count;
     uint32_t coef;     // SK_GENERIC/SK_MFMA: matrix offset (complex elements)
     uint8_t bits[16];  // SK_REG: ascending tile bits; SK_GENERIC/SK_MFMA: tile bit of matrix bit i
+    // Layout of the LDS tile when the step runs: the amplitude of tile index
+    // y sits at LDS index XOR_{b in y} cols[b] (linear over GF(2); CNOTs are
+    // applied by updating cols instead of moving data).
+    uint16_t cols[16];
+    // The same layout pre-composed for SK_REG addressing: thread-index bit i
+    // (the i-th tile bit outside bits[0..4)) contributes tcols[i], register
+    // bit k contributes pcols[k].
+    uint16_t tcols[12];
+    uint16_t pcols[4];
 };
 
 constexpr int kMaxQubits = 40;
@@ -57,6 +67,7 @@ struct PassArgs {
     uint8_t pad;
     uint32_t micro_begin;
     uint32_t micro_count;
+    uint16_t store_cols[16];  // tile layout at the end of the pass (see MicroStep::cols)
 };
 
 struct PlanGate {

@@ -113,7 +113,10 @@ def test_plan_covers_layered_30q_in_few_passes():
     passes, remaps, text = c.plan("c128")
     assert remaps == 0
     assert 5 <= passes <= 40
-    assert f"ops={20 * (2 * n + n - 1)}" in text.splitlines()[0]
+    # 1-qubit runs on a qubit are composed into one op and CNOTs mostly become
+    # layout updates, so there are far fewer register ops than gates
+    ops = int(text.splitlines()[0].split("ops=")[1])
+    assert 0 < ops < 20 * (2 * n + n - 1) // 2
 
 
 def test_plan_remaps_swap_instructions():
