@@ -85,7 +85,11 @@ typedef struct kate_options {
     int32_t dtype;        /* kate_dtype; KATE_C128 is the plain path */
     int32_t tile_qubits;  /* 0 = default per dtype (12 for C128, 13 for C64) */
     int32_t low_qubits;   /* qubits always in a tile (coalesced run = 2^low amplitudes); 0 = default */
-    int32_t use_mfma;     /* dense gates of >= 4 qubits on matrix cores (C64 only) */
+    int32_t use_mfma;     /* C64 only: 1 = dense gates of 4-5 qubits on the matrix cores;
+                             2 = also compose the gates of each 5-qubit window into one
+                             dense gate (FP64 composition) for the matrix cores;
+                             3 = as 2 with split-bf16 products (x = hi + lo, three
+                             products per term) and FP32 accumulation */
     int32_t fold_remaps;  /* reserved, 0 */
     int32_t device;       /* HIP device ordinal */
     int32_t reserved[2];

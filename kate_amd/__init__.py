@@ -240,9 +240,9 @@ class Circuit:
             out.append((kind.value, [int(x) for x in buf[:cnt.value]]))
         return out
 
-    def plan(self, dtype: str = "c128", tile_qubits: int = 0, low_qubits: int = 0) -> Tuple[int, int, str]:
+    def plan(self, dtype: str = "c128", tile_qubits: int = 0, low_qubits: int = 0, mfma: int = 0) -> Tuple[int, int, str]:
         """(gate passes, remap passes, text) of the device plan, without a GPU."""
-        opt = Options({"c128": KATE_C128, "c64": KATE_C64}[dtype], tile_qubits, low_qubits, 0, 0, 0)
+        opt = Options({"c128": KATE_C128, "c64": KATE_C64}[dtype], tile_qubits, low_qubits, mfma, 0, 0)
         p, r = C.c_int32(), C.c_int32()
         buf = C.create_string_buffer(1 << 20)
         _check(lib().kate_circuit_plan(self._h, C.byref(opt), C.byref(p), C.byref(r), buf, 1 << 20), "plan")
@@ -263,11 +263,11 @@ class Simulator:
     """Kate::Simulator on one GPU (HIPcode/simulator.hpp:220-543)."""
 
     def __init__(self, circuit: Circuit, dtype: str = "c128", tile_qubits: int = 0, low_qubits: int = 0,
-                 device: int = 0):
+                 device: int = 0, mfma: int = 0):
         self.n = circuit.n
         self.dtype = {"c128": KATE_C128, "c64": KATE_C64}[dtype]
         self.amp = np.complex128 if self.dtype == KATE_C128 else np.complex64
-        opt = Options(self.dtype, tile_qubits, low_qubits, 0, 0, device)
+        opt = Options(self.dtype, tile_qubits, low_qubits, mfma, 0, device)
         h = VP()
         _check(lib().kate_sim_create_from_circuit(circuit._h, C.byref(opt), C.byref(h)), "kate_sim_create")
         self._h = h

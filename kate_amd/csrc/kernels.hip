@@ -239,11 +239,177 @@ __device__ __forceinline__ void generic_dense(Cx<R>* tile, const MicroStep* __re
 
 // WIDE instantiations also run SK_GENERIC micro-steps; keeping them out of
 // the common variant keeps its register count at two waves per SIMD.
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// Dense K-qubit gate (K = 4 or 5) on the matrix cores, complex64 tile.
+// The gate is a real 2D x 2D matrix (D = 2^K) in the embedding
+// [Yr; Yi] = [[Mr, -Mi], [Mi, Mr]] [Xr; Xi], with the contraction index
+// ordered (gate row g, re/im) so each lane reads one float of X per step.
+// The tile is the 2D x 2^(TB-K) right-hand side; each wave owns whole
+// 32-column blocks, so it reads and overwrites only its own columns.
+// A is pre-arranged on the host in lane order: A[(rb*D + s)*64 + lane] =
+// A'[rb*32 + lane%32][2s + lane/32] for v_mfma_f32_32x32x2f32.
+template <int TB, int K>
+__device__ __forceinline__ void mfma_dense(Cx<float>* tile, const MicroStep* __restrict__ msp,
+                                           const float* __restrict__ A)
+{
+    constexpr int D = 1 << K;
+    constexpr int RB = 2 * D / 32;
+    constexpr int NW = (1 << (TB - 4)) / 64;
+    constexpr int CB = (1 << (TB - K)) / 32;
+    constexpr int CBW = CB / NW;
+    static_assert(CBW >= 1 && CBW * NW == CB, "column blocks must split evenly over the waves");
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t half = (uint32_t)lane >> 5;
+    float* tf = reinterpret_cast<float*>(tile);
+    float a[RB][D];
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+        for (int s = 0; s < D; s++) a[rb][s] = A[(rb * D + s) * 64 + lane];
+    uint32_t gc[K];
+#pragma unroll
+    for (int i = 0; i < K; i++) gc[i] = msp->gcols[i];
+#pragma unroll
+    for (int cbi = 0; cbi < CBW; cbi++) {
+        const uint32_t col = (uint32_t)(wave * CBW + cbi) * 32u + (uint32_t)(lane & 31);
+        uint32_t lq = 0;
+#pragma unroll
+        for (int j = 0; j < TB - K; j++)
+            if ((col >> j) & 1u) lq ^= msp->tcols[j];
+        f32x16 acc[RB];
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[rb][r] = 0.0f;
+#pragma unroll
+        for (int s = 0; s < D; s++) {
+            uint32_t lp = lq;
+#pragma unroll
+            for (int i = 0; i < K; i++)
+                if ((s >> i) & 1) lp ^= gc[i];
+            const float b = tf[2 * swz<float>(lp) + half];
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) acc[rb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[rb][s], b, acc[rb], 0, 0, 0);
+        }
+        // acc[rb][r] holds output row rb*32 + 8(r/4) + 4*half + r%4 of this lane's column
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int o0 = rb * 32 + 8 * (r / 4) + (r % 4);  // row without the lane-half bit (bit 2)
+                const int g0 = o0 % D;
+                uint32_t lp = lq;
+#pragma unroll
+                for (int i = 0; i < K; i++)
+                    if (i != 2 && ((g0 >> i) & 1)) lp ^= gc[i];
+                if (half) lp ^= gc[2];
+                tf[2 * swz<float>(lp) + (uint32_t)(o0 / D)] = acc[rb][r];
+            }
+    }
+}
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint32_t bf16_rn(float f)
+{
+    const uint32_t u = __float_as_uint(f);
+    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
+}
+
+// mfma_dense with split-bf16 products: A = Ahi + Alo (split on the host),
+// B = Bhi + Blo (split here), acc += Ahi Bhi + Ahi Blo + Alo Bhi on
+// v_mfma_f32_32x32x16_bf16 (lane l holds contraction indices 8(l/32)..+7 of
+// its row / column).  Eight consecutive contraction indices are four complex
+// amplitudes of consecutive gate rows, read as four 8-byte LDS loads.
+template <int TB, int K>
+__device__ __forceinline__ void mfma_bf16(Cx<float>* tile, const MicroStep* __restrict__ msp,
+                                          const uint32_t* __restrict__ A)
+{
+    constexpr int D = 1 << K;
+    constexpr int RB = 2 * D / 32;
+    constexpr int T = 2 * D / 16;
+    constexpr int NW = (1 << (TB - 4)) / 64;
+    constexpr int CB = (1 << (TB - K)) / 32;
+    constexpr int CBW = CB / NW;
+    static_assert(CBW >= 1 && CBW * NW == CB, "column blocks must split evenly over the waves");
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t half = (uint32_t)lane >> 5;
+    u32x4 a[RB][T][2];
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+        for (int t = 0; t < T; t++)
+#pragma unroll
+            for (int part = 0; part < 2; part++)
+                a[rb][t][part] = *reinterpret_cast<const u32x4*>(A + ((size_t)((rb * T + t) * 2 + part) * 64 + lane) * 4);
+    uint32_t gc[K];
+#pragma unroll
+    for (int i = 0; i < K; i++) gc[i] = msp->gcols[i];
+    const uint32_t hbit = half ? gc[2] : 0u;
+#pragma unroll
+    for (int cbi = 0; cbi < CBW; cbi++) {
+        const uint32_t col = (uint32_t)(wave * CBW + cbi) * 32u + (uint32_t)(lane & 31);
+        uint32_t lq = 0;
+#pragma unroll
+        for (int j = 0; j < TB - K; j++)
+            if ((col >> j) & 1u) lq ^= msp->tcols[j];
+        f32x16 acc[RB];
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[rb][r] = 0.0f;
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            uint32_t hi[4], lo[4];
+#pragma unroll
+            for (int jj = 0; jj < 4; jj++) {
+                const int g = 8 * t + jj;  // plus 4*half
+                uint32_t lp = lq ^ hbit;
+#pragma unroll
+                for (int i = 0; i < K; i++)
+                    if (i != 2 && ((g >> i) & 1)) lp ^= gc[i];
+                const Cx<float> x = tile[swz<float>(lp)];
+                const uint32_t hr = bf16_rn(x.re), hm = bf16_rn(x.im);
+                const uint32_t lr = bf16_rn(x.re - __uint_as_float(hr << 16));
+                const uint32_t lm = bf16_rn(x.im - __uint_as_float(hm << 16));
+                hi[jj] = hr | (hm << 16);
+                lo[jj] = lr | (lm << 16);
+            }
+            const bf16x8 bh = __builtin_bit_cast(bf16x8, u32x4{hi[0], hi[1], hi[2], hi[3]});
+            const bf16x8 bl = __builtin_bit_cast(bf16x8, u32x4{lo[0], lo[1], lo[2], lo[3]});
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) {
+                const bf16x8 ah = __builtin_bit_cast(bf16x8, a[rb][t][0]);
+                const bf16x8 al = __builtin_bit_cast(bf16x8, a[rb][t][1]);
+                acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[rb], 0, 0, 0);
+                acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[rb], 0, 0, 0);
+                acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[rb], 0, 0, 0);
+            }
+        }
+        float* tf = reinterpret_cast<float*>(tile);
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int o0 = rb * 32 + 8 * (r / 4) + (r % 4);
+                const int g0 = o0 % D;
+                uint32_t lp = lq ^ hbit;
+#pragma unroll
+                for (int i = 0; i < K; i++)
+                    if (i != 2 && ((g0 >> i) & 1)) lp ^= gc[i];
+                tf[2 * swz<float>(lp) + (uint32_t)(o0 / D)] = acc[rb][r];
+            }
+    }
+}
+
 template <typename R, int TB, bool WIDE>
 __global__ __launch_bounds__(1 << (TB - 4)) void k_pass(Cx<R>* __restrict__ state, PassArgs pa,
                                                         const MicroStep* __restrict__ micro,
                                                         const MicroOp* __restrict__ ops,
-                                                        const Cx<R>* __restrict__ coef)
+                                                        const Cx<R>* __restrict__ coef,
+                                                        const uint32_t* __restrict__ raw)
 {
     constexpr int NT = 1 << (TB - 4);
     __shared__ Cx<R> tile[1 << TB];
@@ -292,7 +458,20 @@ __global__ __launch_bounds__(1 << (TB - 4)) void k_pass(Cx<R>* __restrict__ stat
             for (int s = 0; s < 16; s++)
                 tile[swz<R>(b ^ ((s & 1) ? o0 : 0) ^ ((s & 2) ? o1 : 0) ^ ((s & 4) ? o2 : 0) ^ ((s & 8) ? o3 : 0))] = v[s];
         } else if (WIDE) {
-            generic_dense<R, TB>(tile, msp, coef + msp->coef);
+            if constexpr (sizeof(R) == 4) {
+                if (msp->kind == SK_MFMA) {
+                    const float* A = reinterpret_cast<const float*>(raw + msp->coef);
+                    if (msp->nbits == 5) mfma_dense<TB, 5>(reinterpret_cast<Cx<float>*>(tile), msp, A);
+                    else mfma_dense<TB, 4>(reinterpret_cast<Cx<float>*>(tile), msp, A);
+                } else if (msp->kind == SK_MFMA_BF16) {
+                    if (msp->nbits == 5) mfma_bf16<TB, 5>(reinterpret_cast<Cx<float>*>(tile), msp, raw + msp->coef);
+                    else mfma_bf16<TB, 4>(reinterpret_cast<Cx<float>*>(tile), msp, raw + msp->coef);
+                } else {
+                    generic_dense<R, TB>(tile, msp, coef + msp->coef);
+                }
+            } else {
+                generic_dense<R, TB>(tile, msp, coef + msp->coef);
+            }
         }
         __syncthreads();
     }
@@ -511,10 +690,10 @@ int pass_t(void* state, int nq_int, const PassArgs& pa, const DevPlan& dp, hipSt
     const unsigned grid = 1u << (nq_int - TB);
     if (pa.has_wide)
         hipLaunchKernelGGL((k_pass<R, TB, true>), dim3(grid), dim3(1 << (TB - 4)), 0, s, (Cx<R>*)state, pa, dp.micro,
-                           dp.ops, (const Cx<R>*)dp.coef);
+                           dp.ops, (const Cx<R>*)dp.coef, dp.raw);
     else
         hipLaunchKernelGGL((k_pass<R, TB, false>), dim3(grid), dim3(1 << (TB - 4)), 0, s, (Cx<R>*)state, pa, dp.micro,
-                           dp.ops, (const Cx<R>*)dp.coef);
+                           dp.ops, (const Cx<R>*)dp.coef, dp.raw);
     return (int)hipGetLastError();
 }
 

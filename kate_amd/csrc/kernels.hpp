@@ -13,6 +13,7 @@ struct DevPlan {
     const MicroStep* micro;
     const MicroOp* ops;
     const void* coef;  // complex of the state's precision
+    const uint32_t* raw;  // matrix-core operands
 };
 
 // One pass (gate application or remap) over the state.

@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <sstream>
 
 namespace kate {
@@ -225,6 +226,140 @@ struct Builder {
         return "";
     }
 
+    // A dense 2^k x 2^k complex matrix (row-major, interleaved) as the lane-
+    // ordered A operand of the SK_MFMA step (kernels.hip mfma_dense):
+    // A[(rb*D + s)*64 + lane] = A'[rb*32 + lane%32][2s + lane/32] with
+    // A'[i][2g] = Mr[i][g], A'[i][2g+1] = -Mi[i][g] for i < D (real rows) and
+    // A'[D+i][2g] = Mi[i][g], A'[D+i][2g+1] = Mr[i][g] (imaginary rows).
+    uint32_t add_mfma_operand(const double* m, int k)
+    {
+        const int D = 1 << k, RB = 2 * D / 32;
+        std::vector<double> a((size_t)RB * D * 64);
+        for (int rb = 0; rb < RB; rb++)
+            for (int st = 0; st < D; st++)
+                for (int lane = 0; lane < 64; lane++) {
+                    const int i = rb * 32 + lane % 32, kk = 2 * st + lane / 32;
+                    const int g = kk / 2, p = kk % 2;
+                    const int row = i % D;
+                    const double mr = m[2 * ((size_t)row * D + g)], mi = m[2 * ((size_t)row * D + g) + 1];
+                    double v;
+                    if (i < D) v = p == 0 ? mr : -mi;
+                    else v = p == 0 ? mi : mr;
+                    a[(size_t)(rb * D + st) * 64 + lane] = v;
+                }
+        const uint32_t off = (uint32_t)plan.raw.size();
+        for (double v : a) {
+            const float f = (float)v;
+            uint32_t u;
+            std::memcpy(&u, &f, 4);
+            plan.raw.push_back(u);
+        }
+        return off;
+    }
+
+    static uint16_t bf16_rn(float f)
+    {
+        uint32_t u;
+        std::memcpy(&u, &f, 4);
+        return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+    }
+
+    // The same real embedding as add_mfma_operand, split into bf16 hi + lo
+    // parts and laid out for v_mfma_f32_32x32x16_bf16: for row block rb,
+    // K-block t (16 contraction indices) and part (0 hi, 1 lo), lane l holds
+    // A'[rb*32 + l%32][16t + 8(l/32) + j], j = 0..7, as four 32-bit words.
+    uint32_t add_bf16_operand(const double* m, int k)
+    {
+        const int D = 1 << k, RB = 2 * D / 32, T = 2 * D / 16;
+        const uint32_t off = (uint32_t)plan.raw.size();
+        plan.raw.resize(plan.raw.size() + (size_t)RB * T * 2 * 64 * 4);
+        for (int rb = 0; rb < RB; rb++)
+            for (int t = 0; t < T; t++)
+                for (int lane = 0; lane < 64; lane++) {
+                    uint16_t hv[8], lv[8];
+                    for (int j = 0; j < 8; j++) {
+                        const int i = rb * 32 + lane % 32, kk = 16 * t + 8 * (lane / 32) + j;
+                        const int g = kk / 2, p = kk % 2, row = i % D;
+                        const double mr = m[2 * ((size_t)row * D + g)], mi = m[2 * ((size_t)row * D + g) + 1];
+                        const double v = i < D ? (p == 0 ? mr : -mi) : (p == 0 ? mi : mr);
+                        hv[j] = bf16_rn((float)v);
+                        float hf;
+                        const uint32_t hb = (uint32_t)hv[j] << 16;
+                        std::memcpy(&hf, &hb, 4);
+                        lv[j] = bf16_rn((float)(v - (double)hf));
+                    }
+                    for (int part = 0; part < 2; part++) {
+                        const uint16_t* src = part ? lv : hv;
+                        uint32_t* dst = plan.raw.data() + off + ((size_t)((rb * T + t) * 2 + part) * 64 + lane) * 4;
+                        for (int w = 0; w < 4; w++) dst[w] = (uint32_t)src[2 * w] | ((uint32_t)src[2 * w + 1] << 16);
+                    }
+                }
+        return off;
+    }
+
+    // The matrix of gate g on the window bits `wb` (tile bits, ascending),
+    // left-multiplied into `u` (2^w x 2^w, row-major interleaved).
+    void compose_into(std::vector<double>& u, int w, const uint8_t* wb, const PlanGate& g, const int* tile_of)
+    {
+        const int dim = 1 << w;
+        int pos[16];
+        for (int i = 0; i < g.k; i++)
+            for (int j = 0; j < w; j++)
+                if (wb[j] == tile_of[g.q[i]]) pos[i] = j;
+        // gate matrix in its own basis
+        const int gd = 1 << g.k;
+        std::vector<double> gm(2 * (size_t)gd * gd, 0.0);
+        auto set = [&](int r, int c, double re, double im) {
+            gm[2 * ((size_t)r * gd + c)] = re;
+            gm[2 * ((size_t)r * gd + c) + 1] = im;
+        };
+        if (g.id == 0) {
+            std::copy(prog.mat.data() + 2 * g.moff, prog.mat.data() + 2 * g.moff + gm.size(), gm.begin());
+        } else if (g.id == 2) {
+            const double h = 0.7071067811865475;
+            set(0, 0, h, 0); set(0, 1, h, 0); set(1, 0, h, 0); set(1, 1, -h, 0);
+        } else if (g.id == 3) {  // (target, control): bit 0 target, bit 1 control
+            set(0, 0, 1, 0); set(1, 1, 1, 0); set(2, 3, 1, 0); set(3, 2, 1, 0);
+        } else if (g.id == 4) {
+            const double ang = ((double)2 * 3.1415926535897932384626433) / (double)(1ull << g.optarg);
+            set(0, 0, 1, 0); set(1, 1, 1, 0); set(2, 2, 1, 0); set(3, 3, std::cos(ang), std::sin(ang));
+        } else {  // TOFFOLI (c0, c1, target)
+            for (int x = 0; x < 8; x++) set(x, x == 3 ? 7 : (x == 7 ? 3 : x), 1, 0);
+        }
+        std::vector<double> out(u.size(), 0.0);
+        for (int r = 0; r < dim; r++) {
+            int rs = 0;
+            for (int i = 0; i < g.k; i++) rs |= ((r >> pos[i]) & 1) << i;
+            int rest = r;
+            for (int i = 0; i < g.k; i++) rest &= ~(1 << pos[i]);
+            for (int cs = 0; cs < gd; cs++) {
+                const double gr = gm[2 * ((size_t)rs * gd + cs)], gi = gm[2 * ((size_t)rs * gd + cs) + 1];
+                if (gr == 0.0 && gi == 0.0) continue;
+                int mid = rest;
+                for (int i = 0; i < g.k; i++) mid |= ((cs >> i) & 1) << pos[i];
+                for (int c = 0; c < dim; c++) {
+                    const double ur = u[2 * ((size_t)mid * dim + c)], ui = u[2 * ((size_t)mid * dim + c) + 1];
+                    out[2 * ((size_t)r * dim + c)] += gr * ur - gi * ui;
+                    out[2 * ((size_t)r * dim + c) + 1] += gr * ui + gi * ur;
+                }
+            }
+        }
+        u.swap(out);
+    }
+
+    void fill_mfma_cols(MicroStep& ms, const uint16_t* cols)
+    {
+        const int t = plan.tile_bits;
+        uint32_t gmask = 0;
+        for (int i = 0; i < ms.nbits; i++) {
+            ms.gcols[i] = cols[ms.bits[i]];
+            gmask |= 1u << ms.bits[i];
+        }
+        int j = 0;
+        for (int b = 0; b < t; b++)
+            if (!((gmask >> b) & 1u) && j < 12) ms.tcols[j++] = cols[b];
+    }
+
     // A run of 1-qubit gates on one micro bit, composed in FP64.
     struct Held {
         int count = 0;
@@ -334,6 +469,45 @@ struct Builder {
                 ok[i] = g.k <= 4 && !(g.id == 0 && opt.mfma && g.k == 4);
                 lazy[i] = g.id == 3;
             }
+            if (opt.mfma_fuse) {
+                // compose every gate of a window of up to 5 tile bits into
+                // one dense gate for the matrix cores
+                std::vector<uint8_t> ok5(rem.size()), none(rem.size(), 0);
+                for (size_t i = 0; i < rem.size(); i++) ok5[i] = prog.gates[(size_t)rem[i]].k <= 5;
+                uint64_t set = choose_set(tm, ok5, 5, 0, ~0ull);
+                std::vector<int> acc;
+                sweep(tm, ok5, set, ~0ull, nullptr, &acc);
+                if (acc.empty()) return "internal: empty window";
+                int w = popc(set);
+                if (w < 4) {
+                    set = pad_set(set, 4, t);
+                    w = 4;
+                }
+                MicroStep ms{};
+                std::copy(cols, cols + 16, ms.cols);
+                ms.kind = opt.mfma_bf16 ? SK_MFMA_BF16 : SK_MFMA;
+                ms.nbits = (uint8_t)w;
+                int nb = 0;
+                for (int b = 0; b < t; b++)
+                    if ((set >> b) & 1ull) ms.bits[nb++] = (uint8_t)b;
+                std::vector<double> u(2 * (size_t)(1 << w) * (1 << w), 0.0);
+                for (int d = 0; d < (1 << w); d++) u[2 * ((size_t)d * (1 << w) + d)] = 1.0;
+                for (int a : acc) compose_into(u, w, ms.bits, prog.gates[(size_t)rem[(size_t)a]], tile_of);
+                ms.coef = opt.mfma_bf16 ? add_bf16_operand(u.data(), w) : add_mfma_operand(u.data(), w);
+                fill_mfma_cols(ms, cols);
+                plan.micro.push_back(ms);
+                std::vector<int> next;
+                size_t ai = 0;
+                for (size_t i = 0; i < rem.size(); i++) {
+                    if (ai < acc.size() && acc[ai] == (int)i) {
+                        ai++;
+                        continue;
+                    }
+                    next.push_back(rem[i]);
+                }
+                rem.swap(next);
+                continue;
+            }
             const uint64_t set = choose_micro_set(tm, ok, lazy);
             std::vector<int> acc, late;
             sweep_micro(tm, ok, lazy, set, nullptr, &acc, &late);
@@ -358,10 +532,16 @@ struct Builder {
                 // the first remaining gate is one the register path does not take
                 const PlanGate& g = prog.gates[(size_t)rem[0]];
                 if (g.id != 0) return "internal: non-dense gate left out of micro-steps";
-                ms.kind = (opt.mfma && g.k >= 4 && g.k <= 5) ? SK_MFMA : SK_GENERIC;
+                ms.kind = (opt.mfma && g.k >= 4 && g.k <= 5) ? (opt.mfma_bf16 ? SK_MFMA_BF16 : SK_MFMA) : SK_GENERIC;
                 ms.nbits = (uint8_t)g.k;
                 for (int i = 0; i < g.k; i++) ms.bits[i] = (uint8_t)tile_of[g.q[i]];
-                ms.coef = add_coef(prog.mat.data() + 2 * g.moff, (size_t)1 << (2 * g.k));
+                if (ms.kind == SK_MFMA || ms.kind == SK_MFMA_BF16) {
+                    ms.coef = ms.kind == SK_MFMA ? add_mfma_operand(prog.mat.data() + 2 * g.moff, g.k)
+                                                 : add_bf16_operand(prog.mat.data() + 2 * g.moff, g.k);
+                    fill_mfma_cols(ms, cols);
+                } else {
+                    ms.coef = add_coef(prog.mat.data() + 2 * g.moff, (size_t)1 << (2 * g.k));
+                }
                 plan.micro.push_back(ms);
                 rem.erase(rem.begin());
                 continue;

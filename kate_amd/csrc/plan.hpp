@@ -23,7 +23,8 @@ enum OpKind : uint16_t {
 enum StepKind : uint8_t {
     SK_REG = 0,      // 4 tile bits held in registers, ops applied in place
     SK_GENERIC = 1,  // one dense gate of any size, matrix-vector on the LDS tile
-    SK_MFMA = 2,     // one dense gate of 4 or 5 qubits on matrix cores (complex64)
+    SK_MFMA = 2,     // one dense gate of 4 or 5 qubits on matrix cores (complex64, FP32 products)
+    SK_MFMA_BF16 = 3,  // the same with split-bf16 products (x = hi + lo, three products), FP32 accumulation
 };
 
 struct MicroOp {
@@ -38,7 +39,7 @@ struct MicroStep {
     uint8_t pad[2];
     uint32_t op_begin;
     uint32_t op_count;
-    uint32_t coef;     // SK_GENERIC/SK_MFMA: matrix offset (complex elements)
+    uint32_t coef;     // SK_GENERIC: matrix offset (complex elements); SK_MFMA*: operand offset in Plan::raw
     uint8_t bits[16];  // SK_REG: ascending tile bits; SK_GENERIC/SK_MFMA: tile bit of matrix bit i
     // Layout of the LDS tile when the step runs: the amplitude of tile index
     // y sits at LDS index XOR_{b in y} cols[b] (linear over GF(2); CNOTs are
@@ -49,6 +50,9 @@ struct MicroStep {
     // bit k contributes pcols[k].
     uint16_t tcols[12];
     uint16_t pcols[4];
+    // SK_MFMA: layout columns of matrix bit i (gcols) and, in tcols, of the
+    // non-gate tile bits in ascending order (the columns of the contraction).
+    uint16_t gcols[8];
 };
 
 constexpr int kMaxQubits = 40;
@@ -97,7 +101,9 @@ struct PlanOptions {
     int tile_bits = 12;   // qubits per LDS tile
     int low_qubits = 3;   // qubits 0..low-1 are in every gate tile
     int min_qubits = 12;  // the state is padded to at least this many qubits
-    bool mfma = false;    // SK_MFMA for dense gates of 4..5 qubits
+    bool mfma = false;       // SK_MFMA for dense gates of 4..5 qubits (complex64 only)
+    bool mfma_fuse = false;  // also compose the gates of 5-qubit windows into SK_MFMA steps
+    bool mfma_bf16 = false;  // matrix-core steps use split-bf16 products (SK_MFMA_BF16)
 };
 
 struct Plan {
@@ -108,6 +114,7 @@ struct Plan {
     std::vector<MicroStep> micro;
     std::vector<MicroOp> ops;
     std::vector<double> coef;  // interleaved complex128; converted per dtype on upload
+    std::vector<uint32_t> raw; // matrix-core operands, uploaded bit for bit
     int gate_passes = 0;
     int remap_passes = 0;
     std::string describe() const;

@@ -159,6 +159,7 @@ struct kate_sim {
     kate::MicroStep* d_micro = nullptr;
     kate::MicroOp* d_ops = nullptr;
     void* d_coef = nullptr;
+    uint32_t* d_raw = nullptr;
     double* d_scratch = nullptr;
     double* d_out = nullptr;
     uint64_t* d_masks = nullptr;
@@ -190,6 +191,7 @@ void release(kate_sim* s)
     (void)hipFree(s->d_micro);
     (void)hipFree(s->d_ops);
     (void)hipFree(s->d_coef);
+    (void)hipFree(s->d_raw);
     (void)hipFree(s->d_scratch);
     (void)hipFree(s->d_out);
     (void)hipFree(s->d_masks);
@@ -202,7 +204,7 @@ void release(kate_sim* s)
 
 int run_passes(kate_sim* s)
 {
-    kate::DevPlan dp{s->d_micro, s->d_ops, s->d_coef};
+    kate::DevPlan dp{s->d_micro, s->d_ops, s->d_coef, s->d_raw};
     for (const kate::PassArgs& pa : s->plan.passes) {
         int e = kate::launch_pass(s->dtype, s->plan.tile_bits, s->d_state, s->nq_int, pa, dp, s->stream);
         if (e) return fail(KATE_ERR_HIP, std::string("pass launch: ") + hipGetErrorString((hipError_t)e));
@@ -307,7 +309,11 @@ std::string plan_options(const kate_options& opt, kate::PlanOptions& po)
     po.tile_bits = opt.tile_qubits ? opt.tile_qubits : 12;
     po.low_qubits = opt.low_qubits ? opt.low_qubits : (opt.dtype == KATE_C128 ? 3 : 4);
     po.min_qubits = po.tile_bits;
-    po.mfma = false;
+    if (opt.use_mfma && opt.dtype != KATE_C64) return "use_mfma needs KATE_C64 (the FP32 matrix-core path)";
+    if (opt.use_mfma < 0 || opt.use_mfma > 3) return "use_mfma must be 0..3";
+    po.mfma = opt.use_mfma >= 1;
+    po.mfma_fuse = opt.use_mfma >= 2;
+    po.mfma_bf16 = opt.use_mfma == 3;
     if (po.tile_bits < 10 || po.tile_bits > 13) return "tile_qubits must be in 10..13";
     if (po.low_qubits < 0 || po.low_qubits > po.tile_bits) return "low_qubits out of range";
     return "";
@@ -447,6 +453,7 @@ int kate_sim_create(const kate_program* program, const kate_options* options, ka
     step(hipMalloc((void**)&s->d_factors, 4 * (size_t)s->nq_int * sizeof(double)), "hipMalloc(factors)");
     if (rc == KATE_OK) rc = upload(&s->d_micro, s->plan.micro, s->stream);
     if (rc == KATE_OK) rc = upload(&s->d_ops, s->plan.ops, s->stream);
+    if (rc == KATE_OK) rc = upload(&s->d_raw, s->plan.raw, s->stream);
     if (rc == KATE_OK) {
         if (s->dtype == KATE_C128) {
             rc = upload((double**)&s->d_coef, s->plan.coef, s->stream);
@@ -511,7 +518,7 @@ int kate_sim_run_timed(kate_sim* s, float* pass_ms, int32_t* pass_is_remap, int3
     int rc = KATE_OK;
     for (auto& e : ev)
         if (rc == KATE_OK && hipEventCreate(&e) != hipSuccess) rc = fail(KATE_ERR_HIP, "hipEventCreate");
-    kate::DevPlan dp{s->d_micro, s->d_ops, s->d_coef};
+    kate::DevPlan dp{s->d_micro, s->d_ops, s->d_coef, s->d_raw};
     if (rc == KATE_OK && hipEventRecord(ev[0], s->stream) != hipSuccess) rc = fail(KATE_ERR_HIP, "hipEventRecord");
     for (size_t i = 0; i < np && rc == KATE_OK; i++) {
         int e = kate::launch_pass(s->dtype, s->plan.tile_bits, s->d_state, s->nq_int, s->plan.passes[i], dp, s->stream);

@@ -219,3 +219,36 @@ def test_drop_in_binary(gpu, tmp_path):
     tp = np.array([[float(i % 2), 0.0] for i in range(n)])
     ref = O.simulate(n, gates, tp)
     assert np.max(np.abs(amps - ref)) < 1e-12
+
+
+@pytest.mark.parametrize("mode", [1, 3], ids=["fp32", "bf16x3"])
+@pytest.mark.parametrize("k", [4, 5])
+def test_mfma_dense_gates_fp32(gpu, k, mode):
+    """Dense 4- and 5-qubit gates on the matrix cores (complex64, FP32 accumulate)."""
+    say("mfma dense", k)
+    n = 16
+    rng = np.random.default_rng(50 + k)
+    gates = []
+    for i in range(12):
+        qs = tuple(int(x) for x in rng.choice(n, size=k, replace=False))
+        gates.append(W.Gate(0, qs, 0, W.haar_unitary(rng, 1 << k)))
+        gates.append(W.Gate(W.H, (qs[1],)))
+    ref = O.simulate(n, gates)
+    st, _ = gpu_state(K.Circuit(n, gates), dtype="c64", mfma=mode)
+    say("1-F", 1 - fidelity(st, ref), "max-abs", np.max(np.abs(st - ref)))
+    assert 1 - fidelity(st, ref) <= FID32
+    assert np.max(np.abs(st - ref)) < 1e-5
+
+
+@pytest.mark.parametrize("mode", [2, 3], ids=["fp32", "bf16x3"])
+@pytest.mark.parametrize("tile", [12, 13])
+def test_mfma_window_fusion_fp32(gpu, tile, mode):
+    """Every 5-qubit window of gates composed in FP64 and applied on the matrix cores."""
+    say("mfma windows", tile)
+    n = 18
+    gates = W.layered(n, 6, 2) + W.random_mixed(n, 80, 9, max_dense=3)
+    ref = O.simulate(n, gates)
+    st, _ = gpu_state(K.Circuit(n, gates), dtype="c64", mfma=mode, tile_qubits=tile)
+    say("1-F", 1 - fidelity(st, ref), "max-abs", np.max(np.abs(st - ref)))
+    assert 1 - fidelity(st, ref) <= FID32
+    assert np.max(np.abs(st - ref)) < 1e-5
