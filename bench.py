@@ -71,6 +71,11 @@ def main():
     ap.add_argument("--depth", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--serve-qps", type=float, default=4.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the simulate path it measures is Kate's single-GPU call "
@@ -125,10 +130,52 @@ def main():
                              "passes": t32["passes"], "gates_after_fusion": t32["gates"]}
         log(f"fp32 fused: {ms32:.2f} ms/circuit ({ms64 / ms32:.2f}x)")
 
+    if not args.no_batch:
+        out["batched"] = batched(args.batch_count)
+        log(f"batched: {out['batched']}")
+    if not args.no_serve:
+        out["serving"] = serving(args.serve_requests, args.serve_qps)
+        log(f"serving: {out['serving']}")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(min(n, 22), depth, n, len(gates))
         log(f"cpu baseline: {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)
+
+
+def batched(count):
+    """Config C4: `count` Haar brickwork circuits (n log-uniform in [10, 20],
+    depth U{8..32}), complex64, one persistent launch per run under a captured
+    graph; fusion width and tile shape from the autotuner on a 256-circuit
+    sample."""
+    from kate_amd.autotune import autotune_batch, build
+
+    t0 = time.perf_counter()
+    work = W.batch_workload(count, seed=4)
+    (k, tile, sample_ms), table = autotune_batch(work, dtype="c64", ks=(3, 4, 5, 6), tiles=(11, 12, 13), sample=256)
+    circuits = build(work, k)
+    with K.Batch(circuits, dtype="c64", tile_qubits=tile) as b:
+        b.capture()
+        b.run()
+        ms = min(b.run() for _ in range(3))
+        info = b.info()
+    return {"circuits": count, "ms": round(ms, 3), "circuits_per_s": round(count / (ms * 1e-3), 1),
+            "fusion_k": k, "tile_qubits": tile, "tasks": info["tasks"], "persistent_grid": info["grid"],
+            "autotune_table_ms": [[kk, tt, round(mm, 3)] for kk, tt, mm in table],
+            "setup_s": round(time.perf_counter() - t0, 1)}
+
+
+def serving(nreq, qps):
+    """Config C5: Poisson arrivals of layered / QFT circuits, n log-uniform
+    in [12, 30], 8 Z strings each; FP64 plain and complex64 + GateMerger(k=5)."""
+    from kate_amd.serving import request_mix, serve_poisson
+
+    res = {}
+    for dtype, fuse in (("c128", 0), ("c64", 5)):
+        reqs = request_mix(nreq, seed=11, fuse=fuse)
+        r = serve_poisson(reqs, qps, seed=12, dtype=dtype, streams=4, max_active=32)
+        r.pop("results")
+        res["fp64_plain" if dtype == "c128" else "fp32_fused"] = r
+    return res
 
 
 if __name__ == "__main__":

@@ -218,6 +218,72 @@ int kate_sim_create_from_circuit(const kate_circuit* c, const kate_options* opti
 int kate_circuit_plan(const kate_circuit* c, const kate_options* options, int32_t* passes, int32_t* remap_passes,
                       char* buf, size_t len);
 
+/* ---- batches: many small circuits in one persistent launch ----
+ * No counterpart in Kate (one circuit per Simulator).  Each circuit is
+ * planned as kate_sim_create would plan it, with one tile shape for the whole
+ * batch; a run initialises every state to |0...0>, executes every
+ * (circuit, pass, tile) task in one persistent kernel, and reduces <Z_q> for
+ * every qubit of every circuit (FP64, fixed order). */
+typedef struct kate_batch kate_batch;
+int kate_batch_create(const kate_program* programs, int32_t count, const kate_options* options, kate_batch** out);
+int kate_batch_create_from_circuits(const kate_circuit* const* circuits, int32_t count, const kate_options* options,
+                                    kate_batch** out);
+int kate_batch_destroy(kate_batch* batch);
+/* Capture the run sequence (kernels only) into a HIP graph; later runs replay it. */
+int kate_batch_capture(kate_batch* batch);
+/* One run, synchronous; kate_batch_last_ms reports its device time. */
+int kate_batch_run(kate_batch* batch);
+int kate_batch_info(const kate_batch* batch, int32_t* count, int64_t* z_len, int32_t* tasks, int32_t* passes,
+                    int32_t* grid);
+/* <Z_q> of every circuit, concatenated in circuit order (z_len values);
+ * norms (may be NULL) receives each circuit's sum |a|^2. */
+int kate_batch_expect_z(kate_batch* batch, double* out, int64_t len, double* norms);
+int kate_batch_last_ms(const kate_batch* batch, float* ms);
+
+/* ---- serving: continuous batching of simulate requests ----
+ * No counterpart in Kate.  One HBM arena reserved at creation (no allocation
+ * while serving), a pool of streams and one scheduler thread: requests are
+ * planned off the caller's thread, admitted first come first served as soon
+ * as their footprint fits the free arena, and run on the stream pool so that
+ * requests in flight overlap on the device.  Each returns <Z_S> for up to 15
+ * real-qubit masks (FP64 accumulation, normalised). */
+typedef struct kate_server_options {
+    int32_t dtype;
+    int32_t device;
+    int32_t streams;          /* 0 = 4 */
+    int32_t max_active;       /* requests in flight; 0 = 32 */
+    int64_t hbm_budget_bytes; /* arena size; 0 = half the free device memory */
+    int32_t tile_qubits;      /* as kate_options */
+    int32_t low_qubits;
+    int32_t use_mfma;
+    int32_t reserved[3];
+} kate_server_options;
+
+typedef struct kate_request_stats {
+    double submit_s;  /* host steady-clock seconds */
+    double admit_s;
+    double done_s;
+    double latency_ms; /* done - submit */
+    int32_t passes;
+    int32_t nqubits;
+    int64_t footprint_bytes;
+} kate_request_stats;
+
+typedef struct kate_server kate_server;
+int kate_server_create(const kate_server_options* options, kate_server** out);
+/* Finishes the requests in flight, then releases everything. */
+int kate_server_destroy(kate_server* server);
+/* theta_phi: NULL for |0...0>, else the product-state input of
+ * kate_sim_init_product.  Non-blocking; *id names the request. */
+int kate_server_submit(kate_server* server, const kate_program* program, const double* theta_phi,
+                       const uint64_t* zmasks, int32_t nmasks, uint64_t* id);
+int kate_server_submit_circuit(kate_server* server, const kate_circuit* circuit, const double* theta_phi,
+                               const uint64_t* zmasks, int32_t nmasks, uint64_t* id);
+/* Blocks until the request is done; writes min(cap, nmasks) values. */
+int kate_server_wait(kate_server* server, uint64_t id, double* zout, int32_t cap, kate_request_stats* stats);
+int kate_server_pending(kate_server* server, int32_t* queued, int32_t* active, int64_t* completed,
+                        int64_t* largest_free_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,6 +46,18 @@ class Timings(C.Structure):
                 ("gates", C.c_int32), ("bytes_per_pass", C.c_double)]
 
 
+class ServerOptions(C.Structure):
+    _fields_ = [("dtype", C.c_int32), ("device", C.c_int32), ("streams", C.c_int32), ("max_active", C.c_int32),
+                ("hbm_budget_bytes", C.c_int64), ("tile_qubits", C.c_int32), ("low_qubits", C.c_int32),
+                ("use_mfma", C.c_int32), ("reserved", C.c_int32 * 3)]
+
+
+class RequestStats(C.Structure):
+    _fields_ = [("submit_s", C.c_double), ("admit_s", C.c_double), ("done_s", C.c_double),
+                ("latency_ms", C.c_double), ("passes", C.c_int32), ("nqubits", C.c_int32),
+                ("footprint_bytes", C.c_int64)]
+
+
 EXPORTS = [
     "kate_status_string", "kate_last_error", "kate_device_count", "kate_device_info",
     "kate_sim_create", "kate_sim_destroy", "kate_sim_init_basis", "kate_sim_init_product", "kate_sim_run",
@@ -56,7 +68,10 @@ EXPORTS = [
     "kate_circuit_fuse", "kate_circuit_group", "kate_circuit_allocate", "kate_circuit_compile_default",
     "kate_circuit_compile_opt", "kate_circuit_export_qcx", "kate_circuit_info", "kate_circuit_gate",
     "kate_circuit_groups", "kate_circuit_instruction", "kate_circuit_permutations", "kate_sim_create_from_circuit",
-    "kate_circuit_plan",
+    "kate_circuit_plan", "kate_batch_create", "kate_batch_create_from_circuits", "kate_batch_destroy",
+    "kate_batch_capture", "kate_batch_run", "kate_batch_info", "kate_batch_expect_z", "kate_batch_last_ms",
+    "kate_server_create", "kate_server_destroy", "kate_server_submit", "kate_server_submit_circuit",
+    "kate_server_wait", "kate_server_pending",
 ]
 
 VP = C.c_void_p
@@ -113,6 +128,20 @@ def lib():
         L.kate_sim_run_timed.argtypes = [VP, C.POINTER(C.c_float), I32P, C.c_int32, I32P]
         L.kate_plan_describe.argtypes = [VP, C.POINTER(Options), I32P, I32P, C.c_char_p, C.c_size_t]
         L.kate_circuit_plan.argtypes = [VP, C.POINTER(Options), I32P, I32P, C.c_char_p, C.c_size_t]
+        L.kate_batch_create.argtypes = [VP, C.c_int32, C.POINTER(Options), C.POINTER(VP)]
+        L.kate_batch_create_from_circuits.argtypes = [C.POINTER(VP), C.c_int32, C.POINTER(Options), C.POINTER(VP)]
+        L.kate_batch_destroy.argtypes = [VP]
+        L.kate_batch_capture.argtypes = [VP]
+        L.kate_batch_run.argtypes = [VP]
+        L.kate_batch_info.argtypes = [VP, I32P, C.POINTER(C.c_int64), I32P, I32P, I32P]
+        L.kate_batch_expect_z.argtypes = [VP, DP, C.c_int64, DP]
+        L.kate_batch_last_ms.argtypes = [VP, C.POINTER(C.c_float)]
+        L.kate_server_create.argtypes = [C.POINTER(ServerOptions), C.POINTER(VP)]
+        L.kate_server_destroy.argtypes = [VP]
+        L.kate_server_submit.argtypes = [VP, VP, DP, U64P, C.c_int32, C.POINTER(C.c_uint64)]
+        L.kate_server_submit_circuit.argtypes = [VP, VP, DP, U64P, C.c_int32, C.POINTER(C.c_uint64)]
+        L.kate_server_wait.argtypes = [VP, C.c_uint64, DP, C.c_int32, C.POINTER(RequestStats)]
+        L.kate_server_pending.argtypes = [VP, I32P, I32P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         _lib = L
     return _lib
 
@@ -364,3 +393,112 @@ class Simulator:
         buf = C.create_string_buffer(1 << 20)
         _check(lib().kate_sim_describe(self._h, buf, 1 << 20), "describe")
         return buf.value.decode()
+
+
+class Batch:
+    """Many circuits in one persistent launch (kate_batch_*): every state
+    starts at |0...0>; a run returns <Z_q> for every qubit of every circuit."""
+
+    def __init__(self, circuits: Sequence[Circuit], dtype: str = "c64", tile_qubits: int = 0, low_qubits: int = 0,
+                 device: int = 0, mfma: int = 0):
+        self.circuits = list(circuits)
+        self.nq = [c.n for c in self.circuits]
+        arr = (VP * len(self.circuits))(*[c._h for c in self.circuits])
+        opt = Options({"c128": KATE_C128, "c64": KATE_C64}[dtype], tile_qubits, low_qubits, mfma, 0, device)
+        h = VP()
+        _check(lib().kate_batch_create_from_circuits(arr, len(self.circuits), C.byref(opt), C.byref(h)),
+               "kate_batch_create")
+        self._h = h
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.kate_batch_destroy(h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def capture(self):
+        _check(lib().kate_batch_capture(self._h), "kate_batch_capture")
+
+    def run(self) -> float:
+        _check(lib().kate_batch_run(self._h), "kate_batch_run")
+        ms = C.c_float()
+        _check(lib().kate_batch_last_ms(self._h, C.byref(ms)), "kate_batch_last_ms")
+        return float(ms.value)
+
+    def info(self) -> dict:
+        c, t, p, g = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        z = C.c_int64()
+        _check(lib().kate_batch_info(self._h, C.byref(c), C.byref(z), C.byref(t), C.byref(p), C.byref(g)), "info")
+        return {"circuits": c.value, "z_len": z.value, "tasks": t.value, "passes": p.value, "grid": g.value}
+
+    def expect_z(self) -> Tuple[List[np.ndarray], np.ndarray]:
+        """(<Z_q> array per circuit, norms)."""
+        total = sum(self.nq)
+        out = np.zeros(total)
+        norms = np.zeros(len(self.nq))
+        _check(lib().kate_batch_expect_z(self._h, _p(out, DP), total, _p(norms, DP)), "expect_z")
+        res, o = [], 0
+        for n in self.nq:
+            res.append(out[o:o + n].copy())
+            o += n
+        return res, norms
+
+
+class Server:
+    """Continuous-batching request loop (kate_server_*): submit() is
+    non-blocking; wait(id) returns the request's <Z_S> values and stats."""
+
+    def __init__(self, dtype: str = "c128", streams: int = 4, max_active: int = 32, hbm_budget_bytes: int = 0,
+                 tile_qubits: int = 0, low_qubits: int = 0, device: int = 0, mfma: int = 0):
+        opt = ServerOptions({"c128": KATE_C128, "c64": KATE_C64}[dtype], device, streams, max_active,
+                            hbm_budget_bytes, tile_qubits, low_qubits, mfma)
+        h = VP()
+        _check(lib().kate_server_create(C.byref(opt), C.byref(h)), "kate_server_create")
+        self._h = h
+        self._nmasks = {}
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.kate_server_destroy(h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, circuit: Circuit, masks: Sequence[int], theta_phi: Optional[np.ndarray] = None) -> int:
+        m = np.array(masks, dtype=np.uint64)
+        tp = None if theta_phi is None else np.ascontiguousarray(theta_phi, dtype=np.float64).ravel()
+        rid = C.c_uint64()
+        _check(lib().kate_server_submit_circuit(self._h, circuit._h, _p(tp, DP) if tp is not None else None,
+                                                _p(m, U64P), len(m), C.byref(rid)), "kate_server_submit")
+        self._nmasks[rid.value] = len(m)
+        return rid.value
+
+    def wait(self, rid: int) -> Tuple[np.ndarray, dict]:
+        n = self._nmasks.pop(rid, 16)
+        out = np.zeros(max(1, n))
+        st = RequestStats()
+        _check(lib().kate_server_wait(self._h, rid, _p(out, DP), n, C.byref(st)), "kate_server_wait")
+        return out[:n], {f: getattr(st, f) for f, _ in RequestStats._fields_}
+
+    def pending(self) -> dict:
+        q, a = C.c_int32(), C.c_int32()
+        c, f = C.c_int64(), C.c_int64()
+        _check(lib().kate_server_pending(self._h, C.byref(q), C.byref(a), C.byref(c), C.byref(f)), "pending")
+        return {"queued": q.value, "active": a.value, "completed": c.value, "largest_free_bytes": f.value}

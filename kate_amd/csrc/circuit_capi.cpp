@@ -2,6 +2,7 @@
 // (backendagnosticcode/Circuit.hpp, QuantumCircuit.hpp) as implemented by the
 // header-only layer kate_amd/kate/, exported for C and ctypes callers.
 #include <cstring>
+#include <memory>
 #include <string>
 
 #include "Kate.hpp"
@@ -191,6 +192,32 @@ int kate_circuit_plan(const kate_circuit* c, const kate_options* options, int32_
     if (qc.instructions.empty()) qc.compileDefault(0, qc.nqbits);
     Kate::LoweredProgram lp(qc);
     return kate_plan_describe(&lp.prog, options, passes, remap_passes, buf, len);
+}
+
+int kate_batch_create_from_circuits(const kate_circuit* const* circuits, int32_t count, const kate_options* options,
+                                    kate_batch** out)
+{
+    if (!circuits || count < 1 || !out) return cfail(KATE_ERR_ARG, "bad arguments");
+    std::vector<std::unique_ptr<Kate::LoweredProgram>> lowered;
+    std::vector<kate_program> progs;
+    for (int32_t i = 0; i < count; i++) {
+        if (!circuits[i]) return cfail(KATE_ERR_ARG, "null circuit");
+        Kate::QuantumCircuit qc = circuits[i]->c;
+        if (qc.instructions.empty()) qc.compileDefault(0, qc.nqbits);
+        lowered.emplace_back(new Kate::LoweredProgram(qc));
+        progs.push_back(lowered.back()->prog);
+    }
+    return kate_batch_create(progs.data(), count, options, out);
+}
+
+int kate_server_submit_circuit(kate_server* server, const kate_circuit* c, const double* theta_phi,
+                               const uint64_t* zmasks, int32_t nmasks, uint64_t* id)
+{
+    if (!c) return cfail(KATE_ERR_ARG, "null circuit");
+    Kate::QuantumCircuit qc = c->c;
+    if (qc.instructions.empty()) qc.compileDefault(0, qc.nqbits);
+    Kate::LoweredProgram lp(qc);
+    return kate_server_submit(server, &lp.prog, theta_phi, zmasks, nmasks, id);
 }
 
 }  // extern "C"

@@ -53,6 +53,25 @@ __device__ __forceinline__ uint32_t swz(uint32_t j)
     return j ^ ((j >> W) & ((1u << W) - 1u));
 }
 
+// Deterministic reductions: fixed butterfly within a wave, fixed order across waves.
+__device__ __forceinline__ double wave_sum(double x)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+// Sum over the 256 threads of a block, fixed order; result valid in thread 0.
+__device__ __forceinline__ double block_sum(double x, double* lds4)
+{
+    x = wave_sum(x);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) lds4[w] = x;
+    __syncthreads();
+    return ((lds4[0] + lds4[1]) + (lds4[2] + lds4[3]));
+}
+
 // Deposit the low bits of s into the set bits of MASK (compile-time).
 __host__ __device__ constexpr int dep(int mask, int s)
 {
@@ -404,22 +423,18 @@ __device__ __forceinline__ void mfma_bf16(Cx<float>* tile, const MicroStep* __re
     }
 }
 
+// One tile of one pass: load, micro-steps, store.  `blk` is the tile index
+// (its bits are the non-tile qubits rest_pos[]).
 template <typename R, int TB, bool WIDE>
-__global__ __launch_bounds__(1 << (TB - 4)) void k_pass(Cx<R>* __restrict__ state, PassArgs pa,
-                                                        const MicroStep* __restrict__ micro,
-                                                        const MicroOp* __restrict__ ops,
-                                                        const Cx<R>* __restrict__ coef,
-                                                        const uint32_t* __restrict__ raw)
+__device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassArgs& pa, uint64_t blk, Cx<R>* tile,
+                                          const MicroStep* __restrict__ micro, const MicroOp* __restrict__ ops,
+                                          const Cx<R>* __restrict__ coef, const uint32_t* __restrict__ raw)
 {
     constexpr int NT = 1 << (TB - 4);
-    __shared__ Cx<R> tile[1 << TB];
     const uint32_t tid = threadIdx.x;
 
     uint64_t base = 0;
-    {
-        const uint64_t b = blockIdx.x;
-        for (int i = 0; i < pa.nrest; i++) base |= ((b >> i) & 1ull) << pa.rest_pos[i];
-    }
+    for (int i = 0; i < pa.nrest; i++) base |= ((blk >> i) & 1ull) << pa.rest_pos[i];
 
     {   // load
         uint64_t tb = base;
@@ -496,6 +511,135 @@ __global__ __launch_bounds__(1 << (TB - 4)) void k_pass(Cx<R>* __restrict__ stat
     }
 }
 
+template <typename R, int TB, bool WIDE>
+__global__ __launch_bounds__(1 << (TB - 4)) void k_pass(Cx<R>* __restrict__ state, PassArgs pa,
+                                                        const MicroStep* __restrict__ micro,
+                                                        const MicroOp* __restrict__ ops,
+                                                        const Cx<R>* __restrict__ coef,
+                                                        const uint32_t* __restrict__ raw)
+{
+    __shared__ Cx<R> tile[1 << TB];
+    tile_pass<R, TB, WIDE>(state, pa, blockIdx.x, tile, micro, ops, coef, raw);
+}
+
+// ---- many circuits in one persistent launch ---------------------------------
+//
+// Tasks are (circuit, pass, tile), queued by pass level, then circuit, then
+// tile.  A workgroup takes the next task; a task of pass p > 0 first waits
+// until every tile of its circuit's pass p-1 is done.  Tasks are taken in
+// queue order and only wait on earlier ones, which running workgroups hold,
+// so the queue always drains; the wait is still bounded, and a wait that
+// runs out sets *err instead of spinning on.
+template <typename R, int TB, bool WIDE>
+__global__ __launch_bounds__(1 << (TB - 4)) void k_batch(Cx<R>* __restrict__ states,
+                                                         const uint64_t* __restrict__ circ_off,
+                                                         const BatchTask* __restrict__ tasks, uint32_t ntasks,
+                                                         uint32_t* __restrict__ next, uint32_t* __restrict__ done,
+                                                         const PassArgs* __restrict__ passes,
+                                                         const MicroStep* __restrict__ micro,
+                                                         const MicroOp* __restrict__ ops,
+                                                         const Cx<R>* __restrict__ coef,
+                                                         const uint32_t* __restrict__ raw, uint32_t* __restrict__ err)
+{
+    __shared__ Cx<R> tile[1 << TB];
+    __shared__ uint32_t s_task;
+    for (;;) {
+        if (threadIdx.x == 0) s_task = atomicAdd(next, 1u);
+        __syncthreads();
+        const uint32_t t = s_task;
+        __syncthreads();
+        if (t >= ntasks) break;
+        const BatchTask task = tasks[t];
+        if (task.wait_pass != 0xFFFFFFFFu) {
+            if (threadIdx.x == 0) {
+                uint32_t spins = 0;
+                while (__hip_atomic_load(done + task.wait_pass, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+                       task.wait_count) {
+                    __builtin_amdgcn_s_sleep(4);
+                    if (++spins > (1u << 24)) {
+                        atomicOr(err, 1u);
+                        break;
+                    }
+                }
+            }
+            __syncthreads();
+            __threadfence();
+        }
+        tile_pass<R, TB, WIDE>(states + circ_off[task.circuit], passes[task.pass], task.tile, tile, micro, ops, coef,
+                               raw);
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0) atomicAdd(done + task.pass, 1u);
+    }
+}
+
+// Fills n words with v (a kernel, so that it can sit in a captured graph).
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+// |0...0> for every circuit of a batch.
+template <typename R>
+__global__ __launch_bounds__(256) void k_batch_init(Cx<R>* __restrict__ states, uint64_t total,
+                                                    const uint64_t* __restrict__ circ_off, uint32_t count)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride)
+        states[i] = Cx<R>{(R)0, (R)0};
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_batch_ones(Cx<R>* __restrict__ states, const uint64_t* __restrict__ circ_off,
+                                                    uint32_t count)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < count) states[circ_off[c]] = Cx<R>{(R)1, (R)0};
+}
+
+// Per circuit (one workgroup each): norm and <Z_q> for every qubit, FP64,
+// fixed summation order.  Amplitude i = it*256 + tid.
+template <typename R>
+__global__ __launch_bounds__(256) void k_batch_z(const Cx<R>* __restrict__ states, const uint64_t* __restrict__ circ_off,
+                                                 const int32_t* __restrict__ nq, const int32_t* __restrict__ nq_int,
+                                                 const int64_t* __restrict__ zoff, double* __restrict__ z,
+                                                 double* __restrict__ norms)
+{
+    __shared__ double lds[4];
+    const uint32_t c = blockIdx.x, tid = threadIdx.x;
+    const Cx<R>* st = states + circ_off[c];
+    const int n = nq[c];
+    const uint64_t iters = 1ull << (nq_int[c] - 8);
+    double tot = 0.0, hi[24];
+#pragma unroll
+    for (int q = 0; q < 24; q++) hi[q] = 0.0;
+    for (uint64_t it = 0; it < iters; it++) {
+        const Cx<R> a = st[(it << 8) | tid];
+        const double p = (double)a.re * (double)a.re + (double)a.im * (double)a.im;
+        tot += p;
+#pragma unroll
+        for (int q = 0; q < 24; q++)
+            if (q + 8 < n && ((it >> q) & 1ull)) hi[q] += p;
+    }
+    const double total = block_sum(tot, lds);
+    double* out = z + zoff[c];
+    for (int q = 0; q < n; q++) {
+        double s1;
+        if (q < 8) {
+            s1 = block_sum(((tid >> q) & 1u) ? tot : 0.0, lds);
+        } else {
+            double h = 0.0;
+#pragma unroll
+            for (int j = 0; j < 24; j++)
+                if (j == q - 8) h = hi[j];
+            s1 = block_sum(h, lds);
+        }
+        if (tid == 0) out[q] = (total - 2.0 * s1) / total;
+    }
+    if (tid == 0) norms[c] = total;
+}
+
 // ---- initialisation --------------------------------------------------------
 
 template <typename R>
@@ -536,24 +680,6 @@ __global__ __launch_bounds__(256) void k_scale(Cx<R>* __restrict__ st, uint64_t 
 }
 
 // ---- deterministic reductions ------------------------------------------------
-
-__device__ __forceinline__ double wave_sum(double x)
-{
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
-    return x;
-}
-
-// Sum over the 256 threads of a block, fixed order; result valid in thread 0.
-__device__ __forceinline__ double block_sum(double x, double* lds4)
-{
-    x = wave_sum(x);
-    const int w = threadIdx.x >> 6;
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) lds4[w] = x;
-    __syncthreads();
-    return ((lds4[0] + lds4[1]) + (lds4[2] + lds4[3]));
-}
 
 // Level 0 of the amplitude sums: a block reduces 4096 amplitudes (tile index
 // it*256 + tid) to one record {total, S1[bit 0..11]}.
@@ -709,7 +835,57 @@ int pass_r(int tile_bits, void* state, int nq_int, const PassArgs& pa, const Dev
     }
 }
 
+template <typename R, int TB, bool WIDE>
+int batch_t(const BatchArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_batch<R, TB, WIDE>), dim3(a.grid), dim3(1 << (TB - 4)), 0, s, (Cx<R>*)a.states, a.circ_off,
+                       a.tasks, a.ntasks, a.next, a.done, a.passes, a.dp.micro, a.dp.ops, (const Cx<R>*)a.dp.coef,
+                       a.dp.raw, a.err);
+    return (int)hipGetLastError();
+}
+
+template <typename R, bool WIDE>
+int batch_w(const BatchArgs& a, hipStream_t s)
+{
+    switch (a.tile_bits) {
+    case 10: return batch_t<R, 10, WIDE>(a, s);
+    case 11: return batch_t<R, 11, WIDE>(a, s);
+    case 12: return batch_t<R, 12, WIDE>(a, s);
+    case 13: return batch_t<R, 13, WIDE>(a, s);
+    default: return (int)hipErrorInvalidValue;
+    }
+}
+
 }  // namespace
+
+int launch_batch(int dtype, const BatchArgs& a, void* stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    const uint64_t nw = (uint64_t)a.npasses + 2;  // done[] + next + err
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(nw)), dim3(256), 0, s, a.done, (uint64_t)a.npasses, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, a.next, (uint64_t)1, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, a.err, (uint64_t)1, 0u);
+    if (dtype == 0) {
+        hipLaunchKernelGGL(k_batch_init<double>, dim3(grid_for(a.total_amps)), dim3(256), 0, s, (Cx<double>*)a.states,
+                           a.total_amps, a.circ_off, a.count);
+        hipLaunchKernelGGL(k_batch_ones<double>, dim3((a.count + 255) / 256), dim3(256), 0, s, (Cx<double>*)a.states,
+                           a.circ_off, a.count);
+        int e = a.wide ? batch_w<double, true>(a, s) : batch_w<double, false>(a, s);
+        if (e) return e;
+        hipLaunchKernelGGL(k_batch_z<double>, dim3(a.count), dim3(256), 0, s, (const Cx<double>*)a.states, a.circ_off,
+                           a.nq, a.nq_int, a.zoff, a.z, a.norms);
+    } else {
+        hipLaunchKernelGGL(k_batch_init<float>, dim3(grid_for(a.total_amps)), dim3(256), 0, s, (Cx<float>*)a.states,
+                           a.total_amps, a.circ_off, a.count);
+        hipLaunchKernelGGL(k_batch_ones<float>, dim3((a.count + 255) / 256), dim3(256), 0, s, (Cx<float>*)a.states,
+                           a.circ_off, a.count);
+        int e = a.wide ? batch_w<float, true>(a, s) : batch_w<float, false>(a, s);
+        if (e) return e;
+        hipLaunchKernelGGL(k_batch_z<float>, dim3(a.count), dim3(256), 0, s, (const Cx<float>*)a.states, a.circ_off,
+                           a.nq, a.nq_int, a.zoff, a.z, a.norms);
+    }
+    return (int)hipGetLastError();
+}
 
 int launch_pass(int dtype, int tile_bits, void* state, int nq_int, const PassArgs& pa, const DevPlan& dp, void* stream)
 {

@@ -16,6 +16,43 @@ struct DevPlan {
     const uint32_t* raw;  // matrix-core operands
 };
 
+struct BatchTask {
+    uint32_t circuit;
+    uint32_t pass;        // global pass index
+    uint32_t tile;
+    uint32_t wait_pass;   // pass that must be complete first (0xFFFFFFFF: none)
+    uint32_t wait_count;  // its tile count
+    uint32_t pad[3];
+};
+
+struct BatchArgs {
+    void* states;
+    uint64_t total_amps;
+    const uint64_t* circ_off;  // amplitude offset of each circuit's state
+    const int32_t* nq;
+    const int32_t* nq_int;
+    const int64_t* zoff;
+    double* z;
+    double* norms;
+    uint32_t count;
+    const BatchTask* tasks;
+    uint32_t ntasks;
+    uint32_t* next;
+    uint32_t* done;
+    uint32_t npasses;
+    uint32_t* err;
+    const PassArgs* passes;
+    DevPlan dp;
+    int tile_bits;
+    bool wide;
+    unsigned grid;
+};
+
+// Init every state, run every task of the batch in one persistent launch,
+// then <Z_q> and norms per circuit.  All kernels, no memsets: the sequence
+// can be captured in a graph.
+int launch_batch(int dtype, const BatchArgs& a, void* stream);
+
 // One pass (gate application or remap) over the state.
 int launch_pass(int dtype, int tile_bits, void* state, int nq_int, const PassArgs& pa, const DevPlan& dp,
                 void* stream);

@@ -143,8 +143,9 @@ std::string to_program(const kate_program* in, kate::Program& p)
 }  // namespace
 
 namespace kate {
-// Shared with circuit_capi.cpp so kate_last_error() reports either ABI.
+// Shared with the other C-ABI files so kate_last_error() reports any of them.
 int set_error(int code, const std::string& msg) { return fail(code, msg); }
+std::string program_from_abi(const kate_program* in, Program& p) { return to_program(in, p); }
 }  // namespace kate
 
 struct kate_sim {
@@ -302,7 +303,7 @@ uint64_t to_virtual(const kate_sim* s, uint64_t real)
 
 }  // namespace
 
-namespace {
+namespace kate {
 std::string plan_options(const kate_options& opt, kate::PlanOptions& po)
 {
     if (opt.dtype != KATE_C128 && opt.dtype != KATE_C64) return "dtype must be KATE_C128 or KATE_C64";
@@ -318,7 +319,8 @@ std::string plan_options(const kate_options& opt, kate::PlanOptions& po)
     if (po.low_qubits < 0 || po.low_qubits > po.tile_bits) return "low_qubits out of range";
     return "";
 }
-}  // namespace
+}  // namespace kate
+using kate::plan_options;
 
 extern "C" {
 

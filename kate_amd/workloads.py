@@ -177,3 +177,31 @@ def gate_counts(gates: Sequence[Gate]) -> dict:
     for g in gates:
         out[g.id] = out.get(g.id, 0) + 1
     return out
+
+
+def haar_unitaries(rng: np.random.Generator, count: int, dim: int) -> np.ndarray:
+    """``count`` Haar-random U(dim) matrices at once (Mezzadri's QR recipe)."""
+    z = (rng.standard_normal((count, dim, dim)) + 1j * rng.standard_normal((count, dim, dim))) / math.sqrt(2)
+    q, r = np.linalg.qr(z)
+    d = np.diagonal(r, axis1=1, axis2=2)
+    return (q * (d / np.abs(d))[:, None, :]).astype(np.complex128)
+
+
+def batch_workload(count: int, seed: int, nmin: int = 10, nmax: int = 20, dmin: int = 8, dmax: int = 32):
+    """Config-C4 style batch: qubit counts round(exp(U[ln nmin, ln nmax])),
+    depths uniform in [dmin, dmax], Haar 2-qubit brickwork (no long-range
+    layers).  Returns [(n, gates)]."""
+    rng = np.random.default_rng(seed)
+    ns = np.rint(np.exp(rng.uniform(math.log(nmin), math.log(nmax), size=count))).astype(int)
+    depths = rng.integers(dmin, dmax + 1, size=count)
+    need = int(sum(d * (n // 2) for n, d in zip(ns, depths)) + count)
+    mats = haar_unitaries(rng, need, 4)
+    out, k = [], 0
+    for n, d in zip(ns, depths):
+        gates = []
+        for layer in range(int(d)):
+            for a in range(layer % 2, int(n) - 1, 2):
+                gates.append(Gate(DENSE, (a, a + 1), 0, mats[k]))
+                k += 1
+        out.append((int(n), gates))
+    return out

@@ -1,0 +1,80 @@
+"""The continuous-batching server: results against the oracle, each result
+bitwise the same alone and in a mix, admission bounded by the HBM budget."""
+import numpy as np
+import pytest
+
+import kate_amd as K
+from kate_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def make_requests(count, seed, nmin=12, nmax=20):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(count):
+        n = int(rng.integers(nmin, nmax + 1))
+        masks = [int(m) for m in W.zstring_masks(n, 8, seed * 100 + i)]
+        if i % 2 == 0:
+            gates = W.layered(n, int(rng.integers(2, 6)), seed + i)
+            tp = None
+        else:
+            gates = W.qft(n)
+            tp = W.basis_input(n, int(rng.integers(0, 1 << n)))
+        reqs.append((n, gates, tp, masks))
+    return reqs
+
+
+def oracle_z(n, gates, tp, masks):
+    st = O.simulate(n, gates, tp)
+    return O.expect_z(n, st, masks) / O.expect_z(n, st, [0])[0]
+
+
+@pytest.mark.parametrize("dtype,tol", [("c128", 1e-12), ("c64", 2e-6)])
+def test_server_results_match_oracle(gpu, dtype, tol):
+    say("server", dtype)
+    reqs = make_requests(10, 3)
+    with K.Server(dtype=dtype, streams=4, max_active=8, hbm_budget_bytes=2 << 30) as srv:
+        ids = [srv.submit(K.Circuit(n, g), m, tp) for n, g, tp, m in reqs]
+        outs = [srv.wait(i) for i in ids]
+    for (n, g, tp, m), (z, st) in zip(reqs, outs):
+        assert st["nqubits"] == n and st["latency_ms"] > 0
+        assert np.max(np.abs(z - oracle_z(n, g, tp, m))) < tol
+
+
+def test_server_alone_equals_mixed(gpu):
+    say("server alone/mixed")
+    reqs = make_requests(12, 7, 12, 22)
+    probe = reqs[5]
+    with K.Server(dtype="c64", streams=4, max_active=16, hbm_budget_bytes=2 << 30) as srv:
+        alone, _ = srv.wait(srv.submit(K.Circuit(probe[0], probe[1]), probe[3], probe[2]))
+    with K.Server(dtype="c64", streams=4, max_active=16, hbm_budget_bytes=2 << 30) as srv:
+        ids = [srv.submit(K.Circuit(n, g), m, tp) for n, g, tp, m in reqs]
+        mixed = [srv.wait(i)[0] for i in ids]
+    assert np.array_equal(alone, mixed[5])
+
+
+def test_server_admission_by_footprint(gpu):
+    say("server admission")
+    reqs = [(20, W.layered(20, 3, i), None, [1, 2, 3]) for i in range(8)]  # 16 MiB FP64 states
+    with K.Server(dtype="c128", streams=4, max_active=32, hbm_budget_bytes=40 << 20) as srv:
+        ids = [srv.submit(K.Circuit(n, g), m, tp) for n, g, tp, m in reqs]
+        seen_queue = srv.pending()["queued"] > 0
+        outs = [srv.wait(i) for i in ids]
+        assert srv.pending()["completed"] == len(reqs)
+    assert seen_queue  # only two 16 MiB states fit a 40 MiB arena at a time
+    for (n, g, tp, m), (z, _) in zip(reqs, outs):
+        assert np.max(np.abs(z - oracle_z(n, g, tp, m))) < 1e-12
+
+
+def test_server_rejects_request_larger_than_budget(gpu):
+    say("server oversize")
+    with K.Server(dtype="c128", hbm_budget_bytes=8 << 20) as srv:
+        rid = srv.submit(K.Circuit(22, W.layered(22, 1, 0)), [1])
+        with pytest.raises(K.KateError, match="does not fit"):
+            srv.wait(rid)
