@@ -8,7 +8,7 @@ complex128: initialise, run every pass, reduce Kate's proba_state.  The state
 Prints one JSON line.  `value` is the FP64 circuit time in ms.  Beside it:
 the per-pass roofline of the dominant kernel (k_pass, HIP events on the
 simulator's stream), the complex64 + GateMerger(k=5) form of the same
-circuit, and the oracle (plain-C restatement of Kate's CPU simulator) timed
+circuit with its 1 - fidelity against the FP64 state, and the oracle (plain-C restatement of Kate's CPU simulator) timed
 on this host's cores on a bounded sample.
 """
 import argparse
@@ -73,6 +73,9 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-c2", action="store_true")
+    ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--no-qulacs", action="store_true")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-qps", type=float, default=4.0)
@@ -91,12 +94,12 @@ def main():
 
     # ---- FP64 plain path (headline) ----
     c = K.Circuit(n, gates)
-    with K.Simulator(c, dtype="c128") as sim:
-        ms64 = time_executes(sim, args.steps, args.warmup)
-        t = sim.timings()
-        sim.init_basis(0)
-        pass_ms, remap = sim.run_timed()
-        sim.synchronize()
+    sim = K.Simulator(c, dtype="c128")
+    ms64 = time_executes(sim, args.steps, args.warmup)
+    t = sim.timings()
+    sim.init_basis(0)
+    pass_ms, remap = sim.run_timed()
+    sim.synchronize()
     bpp = 2.0 * (1 << max(n, 12)) * 16
     gate_ms = pass_ms[~remap]
     mean_pass = float(gate_ms.mean())
@@ -122,14 +125,27 @@ def main():
 
     # ---- complex64 + GateMerger(k=5) ----
     if not args.no_fp32:
-        c32 = K.Circuit(n, gates).schedule().fuse(5)
-        with K.Simulator(c32, dtype="c64") as sim:
-            ms32 = time_executes(sim, args.steps, args.warmup)
-            t32 = sim.timings()
-        out["fp32_fused"] = {"ms": round(ms32, 3), "speedup_vs_fp64": round(ms64 / ms32, 3),
-                             "passes": t32["passes"], "gates_after_fusion": t32["gates"]}
-        log(f"fp32 fused: {ms32:.2f} ms/circuit ({ms64 / ms32:.2f}x)")
+        for label, circ in (("fp32_fused", K.Circuit(n, gates).schedule().fuse(5)),
+                            ("fp32_plain", K.Circuit(n, gates))):
+            with K.Simulator(circ, dtype="c64") as s32:
+                ms32 = time_executes(s32, args.steps, args.warmup)
+                t32 = s32.timings()
+                # 1 - F against the FP64 state, reduced on the device in FP64.
+                infid = 1.0 - sim.fidelity(s32)
+            out[label] = {"ms": round(ms32, 3), "speedup_vs_fp64": round(ms64 / ms32, 3),
+                          "passes": t32["passes"], "gates": t32["gates"], "one_minus_fidelity": infid}
+            log(f"{label}: {ms32:.2f} ms/circuit ({ms64 / ms32:.2f}x), 1-F = {infid:.3e}")
+    sim.close()
 
+    if not args.no_c2:
+        out["c2_28q"] = c2()
+        log(f"c2: {out['c2_28q']}")
+    if not args.no_c3:
+        out["c3_brickwork"] = c3()
+        log(f"c3: {out['c3_brickwork']}")
+    if not args.no_qulacs:
+        out["qulacs25"] = qulacs()
+        log(f"qulacs25: {out['qulacs25']}")
     if not args.no_batch:
         out["batched"] = batched(args.batch_count)
         log(f"batched: {out['batched']}")
@@ -140,6 +156,79 @@ def main():
         out["cpu_baseline"] = cpu_baseline(min(n, 22), depth, n, len(gates))
         log(f"cpu baseline: {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)
+
+
+def pass_rates(sim, state_bytes):
+    """Per-pass HBM GB/s (2 * state bytes per pass over its event time),
+    split into gate passes and remap passes."""
+    sim.init_basis(0)
+    ms, remap = sim.run_timed()
+    sim.synchronize()
+    out = {}
+    for label, sel in (("gate_passes", ~remap), ("remap_passes", remap)):
+        t = ms[sel]
+        if len(t):
+            gbs = 2.0 * state_bytes / (t * 1e-3) / 1e9
+            out[label] = {"count": int(len(t)), "mean_ms": round(float(t.mean()), 4),
+                          "mean_GBs": round(float(2.0 * state_bytes * len(t) / (t.sum() * 1e-3) / 1e9), 1),
+                          "min_GBs": round(float(gbs.min()), 1), "max_GBs": round(float(gbs.max()), 1)}
+    return out
+
+
+def c2(steps=3):
+    """28-qubit depth-20 layered circuit: FP64 plain beside complex64 +
+    GateMerger(k=5) (VALU and MFMA forms), with max-abs amplitude error and
+    1 - fidelity against the FP64 state."""
+    n, depth = 28, 20
+    gates = W.layered(n, depth, 2)
+    res = {"qubits": n, "depth": depth, "gates": len(gates)}
+    with K.Simulator(K.Circuit(n, gates), dtype="c128") as ref:
+        res["fp64_ms"] = round(time_executes(ref, steps, 1), 3)
+        res["fp64_passes"] = ref.timings()["passes"]
+        for label, mfma in (("fp32_fused_valu", 0), ("fp32_fused_mfma", 1), ("fp32_fused_mfma_bf16x3", 3)):
+            c = K.Circuit(n, gates).schedule().fuse(5)
+            with K.Simulator(c, dtype="c64", mfma=mfma) as s32:
+                ms = time_executes(s32, steps, 1)
+                _, na, nb, mx = ref.compare(s32)
+                res[label] = {"ms": round(ms, 3), "speedup_vs_fp64": round(res["fp64_ms"] / ms, 3),
+                              "max_abs_err": mx, "one_minus_fidelity": 1.0 - ref.fidelity(s32),
+                              "norm_fp32": nb}
+    return res
+
+
+def c3(depth=24):
+    """Haar brickwork, depth 24, long-range (q, q+16 mod n) pairs every 4th
+    layer: 31 qubits complex128 and 32 qubits complex64 (32 GiB each), per-pass
+    HBM GB/s.  The same circuits compiled for a 3-qubit global block (Kate's
+    allocate(3) swap plan) show the in-HBM remap passes that replace its
+    multi-GPU qubit swaps."""
+    res = {}
+    for n, dtype, amp in ((31, "c128", 16), (32, "c64", 8)):
+        gates = W.brickwork(n, depth, 5)
+        state = (1 << n) * amp
+        entry = {"qubits": n, "dtype": dtype, "gates": len(gates), "state_gib": state / 2 ** 30}
+        with K.Simulator(K.Circuit(n, gates), dtype=dtype) as sim:
+            entry["ms"] = round(time_executes(sim, 2, 1), 3)
+            entry["passes"] = pass_rates(sim, state)
+        with K.Simulator(K.Circuit(n, gates).schedule().group(8).allocate(3), dtype=dtype) as sim:
+            entry["swap_plan_ms"] = round(time_executes(sim, 1, 1), 3)
+            entry["swap_plan_passes"] = pass_rates(sim, state)
+        res[f"{n}q_{dtype}"] = entry
+    return res
+
+
+def qulacs(steps=5):
+    """Kate's QulacsBench(25) (1025 gates), FP64, compiled as its main
+    program does (schedule, group(12), compile_default(0)); Kate publishes
+    1035.41 ms for it on one MI200."""
+    n = 25
+    gates = W.qulacs_bench(n)
+    c = K.Circuit(n, gates).schedule().group(12).compile_default(0)
+    with K.Simulator(c, dtype="c128") as sim:
+        ms = time_executes(sim, steps, 2)
+        t = sim.timings()
+    return {"qubits": n, "gates": len(gates), "ms": round(ms, 3), "passes": t["passes"],
+            "kate_mi200_ms": 1035.41, "speedup_vs_kate_mi200": round(1035.41 / ms, 1)}
 
 
 def batched(count):
@@ -158,7 +247,14 @@ def batched(count):
         b.run()
         ms = min(b.run() for _ in range(3))
         info = b.info()
+        z32, _ = b.expect_z()
+    # <Z_i> of every circuit against the same circuits run unfused in FP64.
+    with K.Batch([K.Circuit(n, g) for n, g in work], dtype="c128") as b64:
+        b64.run()
+        z64, _ = b64.expect_z()
+    zerr = max(float(np.abs(np.asarray(a, dtype=np.float64) - np.asarray(r)).max()) for a, r in zip(z32, z64))
     return {"circuits": count, "ms": round(ms, 3), "circuits_per_s": round(count / (ms * 1e-3), 1),
+            "max_z_err_vs_fp64": zerr,
             "fusion_k": k, "tile_qubits": tile, "tasks": info["tasks"], "persistent_grid": info["grid"],
             "autotune_table_ms": [[kk, tt, round(mm, 3)] for kk, tt, mm in table],
             "setup_s": round(time.perf_counter() - t0, 1)}
@@ -169,12 +265,14 @@ def serving(nreq, qps):
     in [12, 30], 8 Z strings each; FP64 plain and complex64 + GateMerger(k=5)."""
     from kate_amd.serving import request_mix, serve_poisson
 
-    res = {}
+    res, zs = {}, {}
     for dtype, fuse in (("c128", 0), ("c64", 5)):
         reqs = request_mix(nreq, seed=11, fuse=fuse)
         r = serve_poisson(reqs, qps, seed=12, dtype=dtype, streams=4, max_active=32)
-        r.pop("results")
+        zs[dtype] = r.pop("results")
         res["fp64_plain" if dtype == "c128" else "fp32_fused"] = r
+    res["fp32_fused"]["max_z_err_vs_fp64"] = max(
+        float(np.abs(np.asarray(a) - np.asarray(b)).max()) for a, b in zip(zs["c64"], zs["c128"]))
     return res
 
 

@@ -155,6 +155,11 @@ int kate_sim_renormalize(kate_sim* sim);
  * Kate leaves them) or real order (real_order = 1, final_inverse_permutation
  * applied).  bytes must be 2^n * sizeof(amplitude). */
 int kate_sim_read_state(kate_sim* sim, void* host, uint64_t bytes, int32_t real_order);
+/* No counterpart in Kate: out[0..1] = <a|b> = sum conj(a_x) b_x, out[2] =
+ * <a|a>, out[3] = <b|b>, all FP64 (fidelity = |<a|b>|^2 / (<a|a><b|b>)),
+ * out[4] = max_x |a_x - b_x|.  The two simulators may differ in precision;
+ * sizes and qubit orders must match. */
+int kate_sim_overlap(kate_sim* a, kate_sim* b, double* out5);
 /* No counterpart in Kate: overwrite the state (virtual order). */
 int kate_sim_write_state(kate_sim* sim, const void* host, uint64_t bytes);
 /* Device address of the state (Simulator::gpu_qbits_states[0]). */

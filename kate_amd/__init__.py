@@ -64,6 +64,7 @@ EXPORTS = [
     "kate_sim_measure_proba", "kate_sim_execute", "kate_sim_expect_z", "kate_sim_norm", "kate_sim_renormalize",
     "kate_sim_read_state", "kate_sim_write_state", "kate_sim_device_state", "kate_sim_last_timings",
     "kate_sim_synchronize", "kate_sim_stream", "kate_sim_describe", "kate_plan_describe", "kate_sim_run_timed",
+    "kate_sim_overlap",
     "kate_circuit_create", "kate_circuit_destroy", "kate_circuit_append", "kate_circuit_schedule",
     "kate_circuit_fuse", "kate_circuit_group", "kate_circuit_allocate", "kate_circuit_compile_default",
     "kate_circuit_compile_opt", "kate_circuit_export_qcx", "kate_circuit_info", "kate_circuit_gate",
@@ -125,6 +126,7 @@ def lib():
         L.kate_circuit_instruction.argtypes = [VP, C.c_int32, I32P, I32P, C.c_int32, I32P]
         L.kate_circuit_permutations.argtypes = [VP, I32P, I32P]
         L.kate_sim_create_from_circuit.argtypes = [VP, C.POINTER(Options), C.POINTER(VP)]
+        L.kate_sim_overlap.argtypes = [VP, VP, DP]
         L.kate_sim_run_timed.argtypes = [VP, C.POINTER(C.c_float), I32P, C.c_int32, I32P]
         L.kate_plan_describe.argtypes = [VP, C.POINTER(Options), I32P, I32P, C.c_char_p, C.c_size_t]
         L.kate_circuit_plan.argtypes = [VP, C.POINTER(Options), I32P, I32P, C.c_char_p, C.c_size_t]
@@ -342,6 +344,21 @@ class Simulator:
         _check(lib().kate_sim_run_timed(self._h, ms.ctypes.data_as(C.POINTER(C.c_float)), _p(rm, I32P), cap,
                                         C.byref(n)), "run_timed")
         return ms[:n.value].astype(np.float64), rm[:n.value].astype(bool)
+
+    def compare(self, other: "Simulator") -> Tuple[complex, float, float, float]:
+        """(<self|other>, <self|self>, <other|other>, max |self_x - other_x|),
+        reduced in FP64 on the device."""
+        out = np.zeros(5)
+        _check(lib().kate_sim_overlap(self._h, other._h, _p(out, DP)), "overlap")
+        return complex(out[0], out[1]), float(out[2]), float(out[3]), float(out[4])
+
+    def overlap(self, other: "Simulator") -> Tuple[complex, float, float]:
+        """(<self|other>, <self|self>, <other|other>) in FP64, on the device."""
+        return self.compare(other)[:3]
+
+    def fidelity(self, other: "Simulator") -> float:
+        ov, na, nb = self.overlap(other)
+        return abs(ov) ** 2 / (na * nb)
 
     def synchronize(self):
         _check(lib().kate_sim_synchronize(self._h), "synchronize")

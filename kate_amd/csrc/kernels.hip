@@ -543,11 +543,22 @@ __global__ __launch_bounds__(1 << (TB - 4)) void k_batch(Cx<R>* __restrict__ sta
 {
     __shared__ Cx<R> tile[1 << TB];
     __shared__ uint32_t s_task;
+    // One thread-0 block per iteration, between two barriers: it counts the
+    // task just finished and fetches the next one.  Two separate thread-0
+    // blocks (count at the end of an iteration, fetch at the start of the
+    // next) sit side by side across the back edge with no barrier between
+    // them; the compiler merges them into a thread-0-only path around the
+    // loop head and leaves the other lanes of wave 0 circling an inner loop
+    // that re-reads the old task index forever.  The index is also read
+    // with readfirstlane so that the exit test is a scalar branch.
+    uint32_t finished = 0xFFFFFFFFu;
     for (;;) {
-        if (threadIdx.x == 0) s_task = atomicAdd(next, 1u);
+        if (threadIdx.x == 0) {
+            if (finished != 0xFFFFFFFFu) atomicAdd(done + finished, 1u);
+            s_task = atomicAdd(next, 1u);
+        }
         __syncthreads();
-        const uint32_t t = s_task;
-        __syncthreads();
+        const uint32_t t = __builtin_amdgcn_readfirstlane(s_task);
         if (t >= ntasks) break;
         const BatchTask task = tasks[t];
         if (task.wait_pass != 0xFFFFFFFFu) {
@@ -556,7 +567,7 @@ __global__ __launch_bounds__(1 << (TB - 4)) void k_batch(Cx<R>* __restrict__ sta
                 while (__hip_atomic_load(done + task.wait_pass, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
                        task.wait_count) {
                     __builtin_amdgcn_s_sleep(4);
-                    if (++spins > (1u << 24)) {
+                    if (++spins > (1u << 20)) {
                         atomicOr(err, 1u);
                         break;
                     }
@@ -567,9 +578,11 @@ __global__ __launch_bounds__(1 << (TB - 4)) void k_batch(Cx<R>* __restrict__ sta
         }
         tile_pass<R, TB, WIDE>(states + circ_off[task.circuit], passes[task.pass], task.tile, tile, micro, ops, coef,
                                raw);
+        // every wave's stores of this tile are out before thread 0 counts it,
+        // and every wave has read s_task before thread 0 overwrites it
         __threadfence();
         __syncthreads();
-        if (threadIdx.x == 0) atomicAdd(done + task.pass, 1u);
+        finished = task.pass;
     }
 }
 
@@ -787,6 +800,44 @@ __global__ __launch_bounds__(256) void k_z0(const Cx<R>* __restrict__ st, const 
     }
 }
 
+// Level 0 of <a|b> = sum conj(a_i) b_i (FP64): a block reduces 4096
+// amplitude pairs to {re, im, |a|^2, |b|^2}.
+// Also folds max |a_i - b_i| into *maxabs (bits of a non-negative double,
+// which order like the values; the max does not depend on the order).
+template <typename RA, typename RB>
+__global__ __launch_bounds__(256) void k_overlap0(const Cx<RA>* __restrict__ a, const Cx<RB>* __restrict__ b,
+                                                  double* __restrict__ part, unsigned long long* __restrict__ maxabs)
+{
+    __shared__ double lds[4];
+    const uint64_t base = (uint64_t)blockIdx.x << 12;
+    double re = 0, im = 0, na = 0, nb = 0, mx = 0;
+#pragma unroll
+    for (int it = 0; it < 16; it++) {
+        const uint64_t i = base + ((uint64_t)it << 8) + threadIdx.x;
+        const Cx<RA> x = a[i];
+        const Cx<RB> y = b[i];
+        const double xr = x.re, xi = x.im, yr = y.re, yi = y.im;
+        re += xr * yr + xi * yi;
+        im += xr * yi - xi * yr;
+        na += xr * xr + xi * xi;
+        nb += yr * yr + yi * yi;
+        const double dr = xr - yr, di = xi - yi;
+        mx = fmax(mx, sqrt(dr * dr + di * di));
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(maxabs, (unsigned long long)__double_as_longlong(mx));
+    double* out = part + (uint64_t)blockIdx.x * 4;
+    double v = block_sum(re, lds);
+    if (threadIdx.x == 0) out[0] = v;
+    v = block_sum(im, lds);
+    if (threadIdx.x == 0) out[1] = v;
+    v = block_sum(na, lds);
+    if (threadIdx.x == 0) out[2] = v;
+    v = block_sum(nb, lds);
+    if (threadIdx.x == 0) out[3] = v;
+}
+
 // rows[count][W] -> out[ceil(count/4096)][W] by fixed-order block sums.
 __global__ __launch_bounds__(256) void k_rowsum(const double* __restrict__ rows, uint64_t count, int W,
                                                 double* __restrict__ out)
@@ -883,6 +934,36 @@ int launch_batch(int dtype, const BatchArgs& a, void* stream)
         if (e) return e;
         hipLaunchKernelGGL(k_batch_z<float>, dim3(a.count), dim3(256), 0, s, (const Cx<float>*)a.states, a.circ_off,
                            a.nq, a.nq_int, a.zoff, a.z, a.norms);
+    }
+    return (int)hipGetLastError();
+}
+
+int launch_overlap(int dtype_a, const void* a, int dtype_b, const void* b, int nq_int, double* d_scratch,
+                   double* d_out, void* stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    if (nq_int < 12) return (int)hipErrorInvalidValue;
+    const uint64_t chunks = 1ull << (nq_int - 12);
+    double* x = chunks == 1 ? d_out : d_scratch;
+    double* y = d_scratch + chunks * 4 + 64;
+    const dim3 g((unsigned)chunks), t(256);
+    unsigned long long* mx = reinterpret_cast<unsigned long long*>(d_out + 4);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(mx), (uint64_t)2, 0u);
+    if (dtype_a == 0 && dtype_b == 0)
+        hipLaunchKernelGGL((k_overlap0<double, double>), g, t, 0, s, (const Cx<double>*)a, (const Cx<double>*)b, x, mx);
+    else if (dtype_a == 0)
+        hipLaunchKernelGGL((k_overlap0<double, float>), g, t, 0, s, (const Cx<double>*)a, (const Cx<float>*)b, x, mx);
+    else if (dtype_b == 0)
+        hipLaunchKernelGGL((k_overlap0<float, double>), g, t, 0, s, (const Cx<float>*)a, (const Cx<double>*)b, x, mx);
+    else
+        hipLaunchKernelGGL((k_overlap0<float, float>), g, t, 0, s, (const Cx<float>*)a, (const Cx<float>*)b, x, mx);
+    uint64_t rows = chunks;
+    while (rows > 1) {
+        const uint64_t blocks = (rows + 4095) / 4096;
+        double* dst = blocks == 1 ? d_out : y;
+        hipLaunchKernelGGL(k_rowsum, dim3((unsigned)blocks), dim3(256), 0, s, x, rows, 4, dst);
+        rows = blocks;
+        std::swap(x, y);
     }
     return (int)hipGetLastError();
 }

@@ -73,4 +73,10 @@ int launch_amp_sums(int dtype, const void* state, int nq_int, double* d_scratch,
 int launch_zsums(int dtype, const void* state, int nq_int, const uint64_t* d_masks, int count, double* d_scratch,
                  double* d_out, void* stream);
 
+// d_out[0..3] = {Re, Im} of <a|b> = sum conj(a_i) b_i, sum |a_i|^2,
+// sum |b_i|^2 (FP64, fixed order); d_out[4] = max |a_i - b_i|.  The two
+// states may differ in precision.
+int launch_overlap(int dtype_a, const void* a, int dtype_b, const void* b, int nq_int, double* d_scratch,
+                   double* d_out, void* stream);
+
 }  // namespace kate

@@ -655,6 +655,23 @@ int kate_sim_write_state(kate_sim* s, const void* host, uint64_t bytes)
     return KATE_OK;
 }
 
+int kate_sim_overlap(kate_sim* a, kate_sim* b, double* out5)
+{
+    if (!a || !b || !out5) return fail(KATE_ERR_ARG, "null argument");
+    if (a->nq != b->nq || a->nq_int != b->nq_int || a->device != b->device)
+        return fail(KATE_ERR_ARG, "overlap needs two simulators of the same size on the same device");
+    if (a->prog.final_inverse_permutation != b->prog.final_inverse_permutation)
+        return fail(KATE_ERR_ARG, "overlap needs the same virtual qubit order in both states");
+    KATE_HIP(hipSetDevice(a->device));
+    KATE_HIP(hipStreamSynchronize(b->stream));
+    int e = kate::launch_overlap(a->dtype, a->d_state, b->dtype, b->d_state, a->nq_int, a->d_scratch, a->d_out,
+                                 a->stream);
+    if (e) return fail(KATE_ERR_HIP, std::string("overlap launch: ") + hipGetErrorString((hipError_t)e));
+    KATE_HIP(hipMemcpyAsync(out5, a->d_out, 5 * sizeof(double), hipMemcpyDeviceToHost, a->stream));
+    KATE_HIP(hipStreamSynchronize(a->stream));
+    return KATE_OK;
+}
+
 int kate_sim_device_state(kate_sim* s, void** dptr)
 {
     if (!s || !dptr) return fail(KATE_ERR_ARG, "null argument");

@@ -252,3 +252,18 @@ def test_mfma_window_fusion_fp32(gpu, tile, mode):
     say("1-F", 1 - fidelity(st, ref), "max-abs", np.max(np.abs(st - ref)))
     assert 1 - fidelity(st, ref) <= FID32
     assert np.max(np.abs(st - ref)) < 1e-5
+
+
+def test_overlap_fidelity_on_device(gpu):
+    say("overlap")
+    n = 20
+    gates = W.layered(n, 6, 3)
+    with K.Simulator(K.Circuit(n, gates), dtype="c128") as a, K.Simulator(K.Circuit(n, gates), dtype="c64") as b:
+        a.execute()
+        b.execute()
+        ov, na, nb, mx = a.compare(b)
+        sa, sb = a.state(real_order=False), b.state(real_order=False).astype(np.complex128)
+        assert abs(mx - np.abs(sa - sb).max()) <= 1e-15
+        assert abs(ov - np.vdot(sa, sb)) < 1e-10
+        assert abs(na - np.vdot(sa, sa).real) < 1e-12 and abs(nb - np.vdot(sb, sb).real) < 1e-10
+        assert 1 - a.fidelity(b) <= FID32
