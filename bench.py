@@ -26,6 +26,12 @@ import kate_amd as K  # noqa: E402
 from kate_amd import workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_pass launch on the default workload (30 qubits, depth 20),
+# from rocprofv3 FETCH_SIZE and WRITE_SIZE runs, each scaled by what the same
+# counter reads on the gate-free passes of the same run (plain tile copies of
+# 16 GiB read + 16 GiB written: factors 2.000 and 1.000): 1.00002 x the
+# algorithmic bytes (profiles/k_pass_f64_30q_layered_hbm_counters.md).
+TRAFFIC_RATIO_30Q_LAYERED = 1.00002
 METRIC = "30q circuit ms + HBM % peak; served circuits/s + p99 at fidelity≥1-1e-6"
 
 
@@ -116,7 +122,8 @@ def main():
                                f"complex128, from |0...0>", "qubits": n, "depth": depth, "gates": len(gates),
                    "state_gib": (1 << n) * 16 / 2 ** 30},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": round(bpp * TRAFFIC_RATIO_30Q_LAYERED) if (n, depth) == (30, 20) else None,
                      "kernel": "k_pass<double,12>", "algorithmic_bytes_per_launch": bpp,
                      "mean_launch_ms": round(mean_pass, 4), "launches": int(len(gate_ms))},
         "passes": int(len(pass_ms)), "init_ms": round(t["init_ms"], 3), "compute_ms": round(t["compute_ms"], 3),
