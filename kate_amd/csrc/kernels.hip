@@ -18,6 +18,9 @@
 // it into FMAs, so results agree with Kate's CPU path to rounding, not bits.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstddef>
+
 #include "kernels.hpp"
 
 namespace kate {
@@ -423,6 +426,50 @@ __device__ __forceinline__ void mfma_bf16(Cx<float>* tile, const MicroStep* __re
     }
 }
 
+// State offset of the first amplitude of tile `blk`: its bits go to the
+// non-tile qubits rest_pos[0..nrest).  Unrolled with constant indices so that
+// every rest_pos[] read is a scalar load and the whole sum stays in SALU;
+// a loop indexed by the run-time nrest turns each read into a dependent
+// vector load.
+__device__ __forceinline__ uint64_t rest_base(const PassArgs& pa, uint64_t blk)
+{
+    uint64_t base = 0;
+    const int nrest = pa.nrest;
+#pragma unroll
+    for (int i = 0; i < kMaxQubits - 10; i++)
+        if (i < nrest) base |= ((blk >> i) & 1ull) << pa.rest_pos[i];
+    return base;
+}
+
+static_assert(offsetof(MicroStep, op_begin) == 4 && offsetof(MicroStep, op_count) == 8 &&
+                  offsetof(MicroStep, tcols) == 64 && offsetof(MicroStep, pcols) == 88,
+              "step_addressing reads MicroStep as 32-bit words");
+
+// Addressing of an SK_REG micro-step for this thread: op range, the four
+// register-bit columns o[] and the thread's base column b.  The descriptor
+// is read as whole 32-bit words at constant offsets (scalar loads); reading
+// its 16-bit fields one by one costs a vector load each.
+template <int TB>
+__device__ __forceinline__ void step_addressing(const MicroStep* __restrict__ msp, uint32_t tid, uint32_t& op_begin,
+                                                uint32_t& op_count, uint32_t (&o)[4], uint32_t& b)
+{
+    const uint32_t* __restrict__ w = reinterpret_cast<const uint32_t*>(msp);
+    op_begin = w[1];
+    op_count = w[2];
+    uint32_t tw[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) tw[k] = w[16 + k];
+    const uint32_t p0 = w[22], p1 = w[23];
+    o[0] = p0 & 0xFFFFu;
+    o[1] = p0 >> 16;
+    o[2] = p1 & 0xFFFFu;
+    o[3] = p1 >> 16;
+    b = 0;
+#pragma unroll
+    for (int i = 0; i < TB - 4; i++)
+        if ((tid >> i) & 1u) b ^= (tw[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu;
+}
+
 // One tile of one pass: load, micro-steps, store.  `blk` is the tile index
 // (its bits are the non-tile qubits rest_pos[]).
 template <typename R, int TB, bool WIDE>
@@ -433,8 +480,7 @@ __device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassA
     constexpr int NT = 1 << (TB - 4);
     const uint32_t tid = threadIdx.x;
 
-    uint64_t base = 0;
-    for (int i = 0; i < pa.nrest; i++) base |= ((blk >> i) & 1ull) << pa.rest_pos[i];
+    const uint64_t base = rest_base(pa, blk);
 
     {   // load
         uint64_t tb = base;
@@ -454,13 +500,10 @@ __device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassA
 
     for (uint32_t mi = 0; mi < pa.micro_count; mi++) {
         const MicroStep* __restrict__ msp = micro + pa.micro_begin + mi;
-        if (msp->kind == SK_REG) {
-            const uint32_t o0 = msp->pcols[0], o1 = msp->pcols[1], o2 = msp->pcols[2], o3 = msp->pcols[3];
-            uint32_t b = 0;
-#pragma unroll
-            for (int i = 0; i < TB - 4; i++)
-                if ((tid >> i) & 1u) b ^= msp->tcols[i];
-            const uint32_t op_begin = msp->op_begin, op_count = msp->op_count;
+        if ((reinterpret_cast<const uint32_t*>(msp)[0] & 0xFFu) == SK_REG) {
+            uint32_t op_begin, op_count, oc[4], b;
+            step_addressing<TB>(msp, tid, op_begin, op_count, oc, b);
+            const uint32_t o0 = oc[0], o1 = oc[1], o2 = oc[2], o3 = oc[3];
             Cx<R> v[16];
 #pragma unroll
             for (int s = 0; s < 16; s++)

@@ -744,7 +744,7 @@ std::string build_plan(const Program& prog, const PlanOptions& opt, Plan& plan)
     return "";
 }
 
-std::string Plan::describe() const
+std::string Plan::describe(bool verbose) const
 {
     std::ostringstream os;
     os << "nq_int=" << nq_int << " tile_bits=" << tile_bits << " passes=" << passes.size() << " (gate " << gate_passes
@@ -762,6 +762,26 @@ std::string Plan::describe() const
         size_t nops = 0;
         for (uint32_t m = 0; m < p.micro_count; m++) nops += micro[p.micro_begin + m].op_count;
         os << " micro=" << p.micro_count << " ops=" << nops << "\n";
+        if (!verbose) continue;
+        static const char* kinds[] = {"reg", "generic", "mfma", "mfma_bf16"};
+        static const char* okinds[] = {"dense", "phase", "h", "cnot", "crk", "toff", "kron"};
+        for (uint32_t m = 0; m < p.micro_count; m++) {
+            const MicroStep& ms = micro[p.micro_begin + m];
+            os << "    step " << m << " " << kinds[ms.kind] << " bits=[";
+            const int nb = ms.kind == SK_REG ? 4 : ms.nbits;
+            for (int b = 0; b < nb; b++) os << (b ? "," : "") << (int)ms.bits[b];
+            os << "]";
+            for (uint32_t o = 0; o < ms.op_count; o++) {
+                const MicroOp& op = ops[ms.op_begin + o];
+                os << " " << okinds[op.code >> 8] << ":" << (op.code & 0xFF);
+            }
+            if (ms.kind == SK_REG) {
+                os << " tcols=";
+                for (int i = 0; i + 4 < p.tile_bits; i++) os << (i ? "," : "") << ms.tcols[i];
+                os << " pcols=" << ms.pcols[0] << "," << ms.pcols[1] << "," << ms.pcols[2] << "," << ms.pcols[3];
+            }
+            os << "\n";
+        }
     }
     return os.str();
 }

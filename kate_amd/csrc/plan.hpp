@@ -117,7 +117,7 @@ struct Plan {
     std::vector<uint32_t> raw; // matrix-core operands, uploaded bit for bit
     int gate_passes = 0;
     int remap_passes = 0;
-    std::string describe() const;
+    std::string describe(bool verbose = false) const;
 };
 
 // Builds the pass plan.  Returns an empty string on success, else the error.

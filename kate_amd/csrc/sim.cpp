@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -394,7 +395,9 @@ int kate_plan_describe(const kate_program* program, const kate_options* options,
     if (passes) *passes = plan.gate_passes;
     if (remap_passes) *remap_passes = plan.remap_passes;
     if (buf && len) {
-        std::string d = plan.describe();
+        // KATE_DESCRIBE_OPS=1 adds every micro-step and its register ops.
+        const char* v = std::getenv("KATE_DESCRIBE_OPS");
+        std::string d = plan.describe(v && v[0] == '1');
         std::strncpy(buf, d.c_str(), len - 1);
         buf[len - 1] = 0;
     }
