@@ -83,7 +83,7 @@ typedef struct kate_program {
 /* Execution options (no counterpart in Kate, whose kernels have one shape). */
 typedef struct kate_options {
     int32_t dtype;        /* kate_dtype; KATE_C128 is the plain path */
-    int32_t tile_qubits;  /* 0 = default per dtype (12 for C128, 13 for C64) */
+    int32_t tile_qubits;  /* 10..13; 0 = 12 */
     int32_t low_qubits;   /* qubits always in a tile (coalesced run = 2^low amplitudes); 0 = default */
     int32_t use_mfma;     /* C64 only: 1 = dense gates of 4-5 qubits on the matrix cores;
                              2 = also compose the gates of each 5-qubit window into one
@@ -92,7 +92,9 @@ typedef struct kate_options {
                              products per term) and FP32 accumulation */
     int32_t fold_remaps;  /* reserved, 0 */
     int32_t device;       /* HIP device ordinal */
-    int32_t reserved[2];
+    int32_t micro_bits;   /* amplitudes per thread in register steps = 2^micro_bits:
+                             3 (C128 only) or 4; 0 = default per dtype */
+    int32_t reserved;
 } kate_options;
 
 /* Wall-clock split of the last execute, as Kate's execute(true) prints it
@@ -261,7 +263,8 @@ typedef struct kate_server_options {
     int32_t tile_qubits;      /* as kate_options */
     int32_t low_qubits;
     int32_t use_mfma;
-    int32_t reserved[3];
+    int32_t micro_bits;       /* as kate_options */
+    int32_t reserved[2];
 } kate_server_options;
 
 typedef struct kate_request_stats {

@@ -37,7 +37,7 @@ class _GateRec(C.Structure):
 class Options(C.Structure):
     _fields_ = [("dtype", C.c_int32), ("tile_qubits", C.c_int32), ("low_qubits", C.c_int32),
                 ("use_mfma", C.c_int32), ("fold_remaps", C.c_int32), ("device", C.c_int32),
-                ("reserved", C.c_int32 * 2)]
+                ("micro_bits", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Timings(C.Structure):
@@ -49,7 +49,7 @@ class Timings(C.Structure):
 class ServerOptions(C.Structure):
     _fields_ = [("dtype", C.c_int32), ("device", C.c_int32), ("streams", C.c_int32), ("max_active", C.c_int32),
                 ("hbm_budget_bytes", C.c_int64), ("tile_qubits", C.c_int32), ("low_qubits", C.c_int32),
-                ("use_mfma", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("use_mfma", C.c_int32), ("micro_bits", C.c_int32), ("reserved", C.c_int32 * 2)]
 
 
 class RequestStats(C.Structure):
@@ -271,9 +271,10 @@ class Circuit:
             out.append((kind.value, [int(x) for x in buf[:cnt.value]]))
         return out
 
-    def plan(self, dtype: str = "c128", tile_qubits: int = 0, low_qubits: int = 0, mfma: int = 0) -> Tuple[int, int, str]:
+    def plan(self, dtype: str = "c128", tile_qubits: int = 0, low_qubits: int = 0, mfma: int = 0,
+             micro_bits: int = 0) -> Tuple[int, int, str]:
         """(gate passes, remap passes, text) of the device plan, without a GPU."""
-        opt = Options({"c128": KATE_C128, "c64": KATE_C64}[dtype], tile_qubits, low_qubits, mfma, 0, 0)
+        opt = Options({"c128": KATE_C128, "c64": KATE_C64}[dtype], tile_qubits, low_qubits, mfma, 0, 0, micro_bits)
         p, r = C.c_int32(), C.c_int32()
         buf = C.create_string_buffer(1 << 20)
         _check(lib().kate_circuit_plan(self._h, C.byref(opt), C.byref(p), C.byref(r), buf, 1 << 20), "plan")
@@ -294,11 +295,11 @@ class Simulator:
     """Kate::Simulator on one GPU (HIPcode/simulator.hpp:220-543)."""
 
     def __init__(self, circuit: Circuit, dtype: str = "c128", tile_qubits: int = 0, low_qubits: int = 0,
-                 device: int = 0, mfma: int = 0):
+                 device: int = 0, mfma: int = 0, micro_bits: int = 0):
         self.n = circuit.n
         self.dtype = {"c128": KATE_C128, "c64": KATE_C64}[dtype]
         self.amp = np.complex128 if self.dtype == KATE_C128 else np.complex64
-        opt = Options(self.dtype, tile_qubits, low_qubits, mfma, 0, device)
+        opt = Options(self.dtype, tile_qubits, low_qubits, mfma, 0, device, micro_bits)
         h = VP()
         _check(lib().kate_sim_create_from_circuit(circuit._h, C.byref(opt), C.byref(h)), "kate_sim_create")
         self._h = h
@@ -417,11 +418,12 @@ class Batch:
     starts at |0...0>; a run returns <Z_q> for every qubit of every circuit."""
 
     def __init__(self, circuits: Sequence[Circuit], dtype: str = "c64", tile_qubits: int = 0, low_qubits: int = 0,
-                 device: int = 0, mfma: int = 0):
+                 device: int = 0, mfma: int = 0, micro_bits: int = 0):
         self.circuits = list(circuits)
         self.nq = [c.n for c in self.circuits]
         arr = (VP * len(self.circuits))(*[c._h for c in self.circuits])
-        opt = Options({"c128": KATE_C128, "c64": KATE_C64}[dtype], tile_qubits, low_qubits, mfma, 0, device)
+        opt = Options({"c128": KATE_C128, "c64": KATE_C64}[dtype], tile_qubits, low_qubits, mfma, 0, device,
+                      micro_bits)
         h = VP()
         _check(lib().kate_batch_create_from_circuits(arr, len(self.circuits), C.byref(opt), C.byref(h)),
                "kate_batch_create")
@@ -475,9 +477,9 @@ class Server:
     non-blocking; wait(id) returns the request's <Z_S> values and stats."""
 
     def __init__(self, dtype: str = "c128", streams: int = 4, max_active: int = 32, hbm_budget_bytes: int = 0,
-                 tile_qubits: int = 0, low_qubits: int = 0, device: int = 0, mfma: int = 0):
+                 tile_qubits: int = 0, low_qubits: int = 0, device: int = 0, mfma: int = 0, micro_bits: int = 0):
         opt = ServerOptions({"c128": KATE_C128, "c64": KATE_C64}[dtype], device, streams, max_active,
-                            hbm_budget_bytes, tile_qubits, low_qubits, mfma)
+                            hbm_budget_bytes, tile_qubits, low_qubits, mfma, micro_bits)
         h = VP()
         _check(lib().kate_server_create(C.byref(opt), C.byref(h)), "kate_server_create")
         self._h = h

@@ -44,6 +44,7 @@ struct kate_batch {
     int device = 0;
     int dtype = KATE_C64;
     int tile_bits = 12;
+    int micro_bits = 4;
     bool wide = false;
     hipStream_t stream = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -112,6 +113,7 @@ kate::BatchArgs args(kate_batch* b)
     a.passes = b->d_passes;
     a.dp = kate::DevPlan{b->d_micro, b->d_ops, b->d_coef, b->d_raw};
     a.tile_bits = b->tile_bits;
+    a.micro_bits = b->micro_bits;
     a.wide = b->wide;
     a.grid = b->grid;
     return a;
@@ -137,6 +139,7 @@ int kate_batch_create(const kate_program* programs, int32_t count, const kate_op
     b->device = opt.device;
     b->dtype = opt.dtype;
     b->tile_bits = po.tile_bits;
+    b->micro_bits = po.micro_bits;
     b->count = (uint32_t)count;
     const size_t amp = opt.dtype == KATE_C128 ? 16 : 8;
 

@@ -89,16 +89,21 @@ __host__ __device__ constexpr int dep(int mask, int s)
 
 __host__ __device__ constexpr int popc4(int m) { return (m & 1) + ((m >> 1) & 1) + ((m >> 2) & 1) + ((m >> 3) & 1); }
 
-// ---- register ops on the 16 amplitudes of a micro-step -------------------
+// ---- register ops on the N = 2^NB amplitudes of a micro-step --------------
+//
+// A micro-step holds N = 8 (NB = 3 micro bits) or N = 16 (NB = 4) amplitudes
+// per thread; register index bit i is micro bit i.  Every op is a template
+// on the array size, and apply_op instantiates only the codes whose bits
+// exist for that size.
 
 // Dense gate on the micro bits of MASK; matrix bit i <-> i-th lowest bit.
-template <int MASK, typename R>
-__device__ __forceinline__ void op_dense(Cx<R> (&v)[16], const Cx<R>* __restrict__ m)
+template <int MASK, typename R, int N>
+__device__ __forceinline__ void op_dense(Cx<R> (&v)[N], const Cx<R>* __restrict__ m)
 {
     constexpr int K = popc4(MASK);
     constexpr int D = 1 << K;
 #pragma unroll
-    for (int rest = 0; rest < 16; rest++) {
+    for (int rest = 0; rest < N; rest++) {
         if (rest & MASK) continue;
         Cx<R> in[D];
 #pragma unroll
@@ -113,21 +118,21 @@ __device__ __forceinline__ void op_dense(Cx<R> (&v)[16], const Cx<R>* __restrict
     }
 }
 
-template <int J, typename R>
-__device__ __forceinline__ void op_phase(Cx<R> (&v)[16], const Cx<R>* __restrict__ m)
+template <int J, typename R, int N>
+__device__ __forceinline__ void op_phase(Cx<R> (&v)[N], const Cx<R>* __restrict__ m)
 {
     const Cx<R> d0 = m[0], d1 = m[1];
 #pragma unroll
-    for (int s = 0; s < 16; s++) v[s] = cmul(((s >> J) & 1) ? d1 : d0, v[s]);
+    for (int s = 0; s < N; s++) v[s] = cmul(((s >> J) & 1) ? d1 : d0, v[s]);
 }
 
 // Kate's Hadamard: a0' = (a0+a1)/sqrt2, a1' = a0' - a1*sqrt2 (GateComputing.hpp:77-78).
-template <int J, typename R>
-__device__ __forceinline__ void op_h(Cx<R> (&v)[16])
+template <int J, typename R, int N>
+__device__ __forceinline__ void op_h(Cx<R> (&v)[N])
 {
     const R inv = (R)0.7071067811865475, s2 = (R)1.4142135623730951;
 #pragma unroll
-    for (int s = 0; s < 16; s++) {
+    for (int s = 0; s < N; s++) {
         if ((s >> J) & 1) continue;
         const Cx<R> a = v[s], b = v[s | (1 << J)];
         const Cx<R> n0{(a.re + b.re) * inv, (a.im + b.im) * inv};
@@ -136,11 +141,11 @@ __device__ __forceinline__ void op_h(Cx<R> (&v)[16])
     }
 }
 
-template <int T, int C, typename R>
-__device__ __forceinline__ void op_cnot(Cx<R> (&v)[16])
+template <int T, int C, typename R, int N>
+__device__ __forceinline__ void op_cnot(Cx<R> (&v)[N])
 {
 #pragma unroll
-    for (int s = 0; s < 16; s++) {
+    for (int s = 0; s < N; s++) {
         if (((s >> C) & 1) == 0 || ((s >> T) & 1)) continue;
         const Cx<R> t = v[s];
         v[s] = v[s | (1 << T)];
@@ -148,20 +153,20 @@ __device__ __forceinline__ void op_cnot(Cx<R> (&v)[16])
     }
 }
 
-template <int A, int B, typename R>
-__device__ __forceinline__ void op_crk(Cx<R> (&v)[16], const Cx<R>* __restrict__ m)
+template <int A, int B, typename R, int N>
+__device__ __forceinline__ void op_crk(Cx<R> (&v)[N], const Cx<R>* __restrict__ m)
 {
     const Cx<R> ph = m[0];
 #pragma unroll
-    for (int s = 0; s < 16; s++)
+    for (int s = 0; s < N; s++)
         if (((s >> A) & 1) && ((s >> B) & 1)) v[s] = cmul(v[s], ph);
 }
 
-template <int T, int C0, int C1, typename R>
-__device__ __forceinline__ void op_toff(Cx<R> (&v)[16])
+template <int T, int C0, int C1, typename R, int N>
+__device__ __forceinline__ void op_toff(Cx<R> (&v)[N])
 {
 #pragma unroll
-    for (int s = 0; s < 16; s++) {
+    for (int s = 0; s < N; s++) {
         if (((s >> C0) & 1) == 0 || ((s >> C1) & 1) == 0 || ((s >> T) & 1)) continue;
         const Cx<R> t = v[s];
         v[s] = v[s | (1 << T)];
@@ -171,42 +176,45 @@ __device__ __forceinline__ void op_toff(Cx<R> (&v)[16])
 
 // Independent 2x2 gates on every micro bit of MASK (one matrix per bit, in
 // ascending bit order), applied one after the other.
-template <int MASK, typename R>
-__device__ __forceinline__ void op_kron(Cx<R> (&v)[16], const Cx<R>* __restrict__ m)
+template <int MASK, typename R, int N>
+__device__ __forceinline__ void op_kron(Cx<R> (&v)[N], const Cx<R>* __restrict__ m)
 {
     int k = 0;
     if (MASK & 1) op_dense<1>(v, m + 4 * k++);
     if (MASK & 2) op_dense<2>(v, m + 4 * k++);
     if (MASK & 4) op_dense<4>(v, m + 4 * k++);
-    if (MASK & 8) op_dense<8>(v, m + 4 * k++);
+    if constexpr (N > 8)
+        if (MASK & 8) op_dense<8>(v, m + 4 * k++);
 }
 
 #define KATE_OPC(kind, var) (((kind) << 8) | (var))
 
-template <typename R>
-__device__ __forceinline__ void apply_op(Cx<R> (&v)[16], uint32_t code, const Cx<R>* __restrict__ cf)
+template <typename R, int N>
+__device__ __forceinline__ void apply_op(Cx<R> (&v)[N], uint32_t code, const Cx<R>* __restrict__ cf)
 {
     switch (code) {
 #define D(M) \
-    case KATE_OPC(OK_DENSE, M): op_dense<M>(v, cf); break; \
-    case KATE_OPC(OK_KRON, M): op_kron<M>(v, cf); break;
+    case KATE_OPC(OK_DENSE, M): if constexpr ((M) < N) op_dense<M>(v, cf); break; \
+    case KATE_OPC(OK_KRON, M): if constexpr ((M) < N) op_kron<M>(v, cf); break;
         D(1) D(2) D(3) D(4) D(5) D(6) D(7) D(8) D(9) D(10) D(11) D(12) D(13) D(14) D(15)
 #undef D
 #define P(J) \
-    case KATE_OPC(OK_PHASE, J): op_phase<J>(v, cf); break; \
-    case KATE_OPC(OK_H, J): op_h<J>(v); break;
+    case KATE_OPC(OK_PHASE, J): if constexpr ((1 << (J)) < N) op_phase<J>(v, cf); break; \
+    case KATE_OPC(OK_H, J): if constexpr ((1 << (J)) < N) op_h<J>(v); break;
         P(0) P(1) P(2) P(3)
 #undef P
 #define CN(T, C) \
-    case KATE_OPC(OK_CNOT, (T) * 4 + (C)): op_cnot<T, C>(v); break;
+    case KATE_OPC(OK_CNOT, (T) * 4 + (C)): if constexpr ((1 << (T)) < N && (1 << (C)) < N) op_cnot<T, C>(v); break;
         CN(0, 1) CN(0, 2) CN(0, 3) CN(1, 0) CN(1, 2) CN(1, 3) CN(2, 0) CN(2, 1) CN(2, 3) CN(3, 0) CN(3, 1) CN(3, 2)
 #undef CN
 #define CR(A, B) \
-    case KATE_OPC(OK_CRK, (A) * 4 + (B)): op_crk<A, B>(v, cf); break;
+    case KATE_OPC(OK_CRK, (A) * 4 + (B)): if constexpr ((1 << (A)) < N && (1 << (B)) < N) op_crk<A, B>(v, cf); break;
         CR(0, 1) CR(0, 2) CR(0, 3) CR(1, 2) CR(1, 3) CR(2, 3)
 #undef CR
 #define TF(T, A, B) \
-    case KATE_OPC(OK_TOFF, (T) * 16 + (A) * 4 + (B)): op_toff<T, A, B>(v); break;
+    case KATE_OPC(OK_TOFF, (T) * 16 + (A) * 4 + (B)): \
+        if constexpr ((1 << (T)) < N && (1 << (A)) < N && (1 << (B)) < N) op_toff<T, A, B>(v); \
+        break;
         TF(0, 1, 2) TF(0, 1, 3) TF(0, 2, 3) TF(1, 0, 2) TF(1, 0, 3) TF(1, 2, 3)
         TF(2, 0, 1) TF(2, 0, 3) TF(2, 1, 3) TF(3, 0, 1) TF(3, 0, 2) TF(3, 1, 2)
 #undef TF
@@ -226,21 +234,22 @@ __device__ __forceinline__ uint32_t layout(uint32_t y, const uint16_t* __restric
     return p;
 }
 
-// One dense gate of any width on the LDS tile: every thread computes its 16
-// outputs from LDS, then (after a barrier) writes them.
-template <typename R, int TB>
+// One dense gate of any width on the LDS tile: every thread computes its
+// 2^NB outputs from LDS, then (after a barrier) writes them.
+template <typename R, int TB, int NB>
 __device__ __forceinline__ void generic_dense(Cx<R>* tile, const MicroStep* __restrict__ msp,
                                               const Cx<R>* __restrict__ m)
 {
     const MicroStep& ms = *msp;
-    constexpr int NT = 1 << (TB - 4);
+    constexpr int NT = 1 << (TB - NB);
+    constexpr int N = 1 << NB;
     const int k = ms.nbits;
     const uint32_t dim = 1u << k;
     uint32_t gmask = 0;
     for (int i = 0; i < k; i++) gmask |= 1u << ms.bits[i];
-    Cx<R> out[16];
+    Cx<R> out[N];
 #pragma unroll
-    for (int e = 0; e < 16; e++) {
+    for (int e = 0; e < N; e++) {
         const uint32_t j = (uint32_t)e * NT + threadIdx.x;
         uint32_t row = 0;
         for (int i = 0; i < k; i++) row |= ((j >> ms.bits[i]) & 1u) << i;
@@ -256,7 +265,7 @@ __device__ __forceinline__ void generic_dense(Cx<R>* tile, const MicroStep* __re
     }
     __syncthreads();
 #pragma unroll
-    for (int e = 0; e < 16; e++) tile[swz<R>(layout<TB>((uint32_t)e * NT + threadIdx.x, ms.cols))] = out[e];
+    for (int e = 0; e < N; e++) tile[swz<R>(layout<TB>((uint32_t)e * NT + threadIdx.x, ms.cols))] = out[e];
 }
 
 // WIDE instantiations also run SK_GENERIC micro-steps; keeping them out of
@@ -441,60 +450,81 @@ __device__ __forceinline__ uint64_t rest_base(const PassArgs& pa, uint64_t blk)
     return base;
 }
 
-static_assert(offsetof(MicroStep, op_begin) == 4 && offsetof(MicroStep, op_count) == 8 &&
-                  offsetof(MicroStep, tcols) == 64 && offsetof(MicroStep, pcols) == 88,
+static_assert(offsetof(MicroStep, op0_code) == 2 && offsetof(MicroStep, coef) == 12 &&
+                  offsetof(MicroStep, op_begin) == 4 && offsetof(MicroStep, op_count) == 8 &&
+                  offsetof(MicroStep, tbyte) == 112 && offsetof(MicroStep, pbyte) == 160,
               "step_addressing reads MicroStep as 32-bit words");
 
-// Addressing of an SK_REG micro-step for this thread: op range, the four
-// register-bit columns o[] and the thread's base column b.  The descriptor
-// is read as whole 32-bit words at constant offsets (scalar loads); reading
-// its 16-bit fields one by one costs a vector load each.
-template <int TB>
+// Addressing of an SK_REG micro-step for this thread: op range, the LDS
+// byte offsets o[] of the register bits and the thread's base offset b (all
+// already swizzled: the swizzle is linear, so offsets combine by XOR).  The
+// descriptor is read as 32-bit words at constant offsets (scalar loads).
+template <int TB, int NB>
 __device__ __forceinline__ void step_addressing(const MicroStep* __restrict__ msp, uint32_t tid, uint32_t& op_begin,
                                                 uint32_t& op_count, uint32_t (&o)[4], uint32_t& b)
 {
     const uint32_t* __restrict__ w = reinterpret_cast<const uint32_t*>(msp);
     op_begin = w[1];
     op_count = w[2];
-    uint32_t tw[6];
 #pragma unroll
-    for (int k = 0; k < 6; k++) tw[k] = w[16 + k];
-    const uint32_t p0 = w[22], p1 = w[23];
-    o[0] = p0 & 0xFFFFu;
-    o[1] = p0 >> 16;
-    o[2] = p1 & 0xFFFFu;
-    o[3] = p1 >> 16;
+    for (int k = 0; k < 4; k++) o[k] = w[40 + k];
     b = 0;
 #pragma unroll
-    for (int i = 0; i < TB - 4; i++)
-        if ((tid >> i) & 1u) b ^= (tw[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu;
+    for (int i = 0; i < TB - NB; i++)
+        if ((tid >> i) & 1u) b ^= w[28 + i];
+}
+
+template <typename R>
+__device__ __forceinline__ Cx<R>& lds_at(Cx<R>* tile, uint32_t byte_off)
+{
+    return *reinterpret_cast<Cx<R>*>(reinterpret_cast<char*>(tile) + byte_off);
+}
+
+// LDS offset of register amplitude s from the columns of the micro bits.
+template <int NB>
+__device__ __forceinline__ uint32_t reg_offset(int s, const uint32_t (&o)[4])
+{
+    uint32_t x = 0;
+#pragma unroll
+    for (int k = 0; k < NB; k++)
+        if ((s >> k) & 1) x ^= o[k];
+    return x;
 }
 
 // One tile of one pass: load, micro-steps, store.  `blk` is the tile index
-// (its bits are the non-tile qubits rest_pos[]).
-template <typename R, int TB, bool WIDE>
+// (its bits are the non-tile qubits rest_pos[]).  Each of the 2^(TB-NB)
+// threads holds N = 2^NB amplitudes; the register bits are tile bits
+// TB-NB..TB-1 when loading and storing.
+template <typename R, int TB, bool WIDE, int NB>
 __device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassArgs& pa, uint64_t blk, Cx<R>* tile,
                                           const MicroStep* __restrict__ micro, const MicroOp* __restrict__ ops,
                                           const Cx<R>* __restrict__ coef, const uint32_t* __restrict__ raw)
 {
-    constexpr int NT = 1 << (TB - 4);
+    constexpr int NT = 1 << (TB - NB);
+    constexpr int N = 1 << NB;
     const uint32_t tid = threadIdx.x;
 
     const uint64_t base = rest_base(pa, blk);
 
     {   // load
         uint64_t tb = base;
-        for (int b = 0; b < TB - 4; b++) tb |= (uint64_t)((tid >> b) & 1u) << pa.load_pos[b];
-        const uint64_t m0 = 1ull << pa.load_pos[TB - 4], m1 = 1ull << pa.load_pos[TB - 3];
-        const uint64_t m2 = 1ull << pa.load_pos[TB - 2], m3 = 1ull << pa.load_pos[TB - 1];
-        Cx<R> r[16];
 #pragma unroll
-        for (int i = 0; i < 16; i++) {
-            const uint64_t off = tb + ((i & 1) ? m0 : 0) + ((i & 2) ? m1 : 0) + ((i & 4) ? m2 : 0) + ((i & 8) ? m3 : 0);
+        for (int b = 0; b < TB - NB; b++) tb |= (uint64_t)((tid >> b) & 1u) << pa.load_pos[b];
+        uint64_t m[NB];
+#pragma unroll
+        for (int k = 0; k < NB; k++) m[k] = 1ull << pa.load_pos[TB - NB + k];
+        Cx<R> r[N];
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            uint64_t off = tb;
+#pragma unroll
+            for (int k = 0; k < NB; k++)
+                if ((i >> k) & 1) off += m[k];
             r[i] = state[off];
         }
+        const uint32_t tsw = swz<R>(tid) * (uint32_t)sizeof(Cx<R>);
 #pragma unroll
-        for (int i = 0; i < 16; i++) tile[swz<R>((uint32_t)i * NT + tid)] = r[i];
+        for (int i = 0; i < N; i++) lds_at(tile, tsw ^ (swz<R>((uint32_t)i * NT) * (uint32_t)sizeof(Cx<R>))) = r[i];
     }
     __syncthreads();
 
@@ -502,21 +532,26 @@ __device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassA
         const MicroStep* __restrict__ msp = micro + pa.micro_begin + mi;
         if ((reinterpret_cast<const uint32_t*>(msp)[0] & 0xFFu) == SK_REG) {
             uint32_t op_begin, op_count, oc[4], b;
-            step_addressing<TB>(msp, tid, op_begin, op_count, oc, b);
-            const uint32_t o0 = oc[0], o1 = oc[1], o2 = oc[2], o3 = oc[3];
-            Cx<R> v[16];
+            step_addressing<TB, NB>(msp, tid, op_begin, op_count, oc, b);
+            const uint32_t w0 = reinterpret_cast<const uint32_t*>(msp)[0];
+            const uint32_t code0 = w0 >> 16, coef0 = reinterpret_cast<const uint32_t*>(msp)[3];
+            Cx<R> v[N];
 #pragma unroll
-            for (int s = 0; s < 16; s++)
-                v[s] = tile[swz<R>(b ^ ((s & 1) ? o0 : 0) ^ ((s & 2) ? o1 : 0) ^ ((s & 4) ? o2 : 0) ^ ((s & 8) ? o3 : 0))];
+            for (int s = 0; s < N; s++) v[s] = lds_at(tile, b ^ reg_offset<NB>(s, oc));
+            // the first op comes with the step (no load that waits on the step)
             for (uint32_t o = 0; o < op_count; o++) {
-                const MicroOp op = ops[op_begin + o];
-                apply_op<R>(v, op.code, coef + op.coef);
+                uint32_t code = code0, cofs = coef0;
+                if (o) {
+                    const MicroOp op = ops[op_begin + o];
+                    code = op.code;
+                    cofs = op.coef;
+                }
+                apply_op<R>(v, code, coef + cofs);
             }
 #pragma unroll
-            for (int s = 0; s < 16; s++)
-                tile[swz<R>(b ^ ((s & 1) ? o0 : 0) ^ ((s & 2) ? o1 : 0) ^ ((s & 4) ? o2 : 0) ^ ((s & 8) ? o3 : 0))] = v[s];
+            for (int s = 0; s < N; s++) lds_at(tile, b ^ reg_offset<NB>(s, oc)) = v[s];
         } else if (WIDE) {
-            if constexpr (sizeof(R) == 4) {
+            if constexpr (sizeof(R) == 4 && NB == 4) {
                 if (msp->kind == SK_MFMA) {
                     const float* A = reinterpret_cast<const float*>(raw + msp->coef);
                     if (msp->nbits == 5) mfma_dense<TB, 5>(reinterpret_cast<Cx<float>*>(tile), msp, A);
@@ -525,10 +560,10 @@ __device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassA
                     if (msp->nbits == 5) mfma_bf16<TB, 5>(reinterpret_cast<Cx<float>*>(tile), msp, raw + msp->coef);
                     else mfma_bf16<TB, 4>(reinterpret_cast<Cx<float>*>(tile), msp, raw + msp->coef);
                 } else {
-                    generic_dense<R, TB>(tile, msp, coef + msp->coef);
+                    generic_dense<R, TB, NB>(tile, msp, coef + msp->coef);
                 }
             } else {
-                generic_dense<R, TB>(tile, msp, coef + msp->coef);
+                generic_dense<R, TB, NB>(tile, msp, coef + msp->coef);
             }
         }
         __syncthreads();
@@ -537,32 +572,38 @@ __device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassA
     {   // store, reading the tile through its final layout
         uint64_t tb = base;
         uint32_t lt = 0;
-        for (int b = 0; b < TB - 4; b++) {
-            tb |= (uint64_t)((tid >> b) & 1u) << pa.store_pos[b];
-            if ((tid >> b) & 1u) lt ^= pa.store_cols[b];
-        }
-        const uint64_t m0 = 1ull << pa.store_pos[TB - 4], m1 = 1ull << pa.store_pos[TB - 3];
-        const uint64_t m2 = 1ull << pa.store_pos[TB - 2], m3 = 1ull << pa.store_pos[TB - 1];
-        const uint32_t l0 = pa.store_cols[TB - 4], l1 = pa.store_cols[TB - 3], l2 = pa.store_cols[TB - 2],
-                       l3 = pa.store_cols[TB - 1];
 #pragma unroll
-        for (int i = 0; i < 16; i++) {
-            const uint64_t off = tb + ((i & 1) ? m0 : 0) + ((i & 2) ? m1 : 0) + ((i & 4) ? m2 : 0) + ((i & 8) ? m3 : 0);
-            const uint32_t li = lt ^ ((i & 1) ? l0 : 0) ^ ((i & 2) ? l1 : 0) ^ ((i & 4) ? l2 : 0) ^ ((i & 8) ? l3 : 0);
-            state[off] = tile[swz<R>(li)];
+        for (int b = 0; b < TB - NB; b++) {
+            tb |= (uint64_t)((tid >> b) & 1u) << pa.store_pos[b];
+            if ((tid >> b) & 1u) lt ^= pa.store_byte[b];
+        }
+        uint64_t m[NB];
+        uint32_t l[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < NB; k++) {
+            m[k] = 1ull << pa.store_pos[TB - NB + k];
+            l[k] = pa.store_byte[TB - NB + k];
+        }
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            uint64_t off = tb;
+#pragma unroll
+            for (int k = 0; k < NB; k++)
+                if ((i >> k) & 1) off += m[k];
+            state[off] = lds_at(tile, lt ^ reg_offset<NB>(i, l));
         }
     }
 }
 
-template <typename R, int TB, bool WIDE>
-__global__ __launch_bounds__(1 << (TB - 4)) void k_pass(Cx<R>* __restrict__ state, PassArgs pa,
-                                                        const MicroStep* __restrict__ micro,
-                                                        const MicroOp* __restrict__ ops,
-                                                        const Cx<R>* __restrict__ coef,
-                                                        const uint32_t* __restrict__ raw)
+template <typename R, int TB, bool WIDE, int NB>
+__global__ __launch_bounds__(1 << (TB - NB)) void k_pass(Cx<R>* __restrict__ state, PassArgs pa,
+                                                         const MicroStep* __restrict__ micro,
+                                                         const MicroOp* __restrict__ ops,
+                                                         const Cx<R>* __restrict__ coef,
+                                                         const uint32_t* __restrict__ raw)
 {
     __shared__ Cx<R> tile[1 << TB];
-    tile_pass<R, TB, WIDE>(state, pa, blockIdx.x, tile, micro, ops, coef, raw);
+    tile_pass<R, TB, WIDE, NB>(state, pa, blockIdx.x, tile, micro, ops, coef, raw);
 }
 
 // ---- many circuits in one persistent launch ---------------------------------
@@ -573,8 +614,8 @@ __global__ __launch_bounds__(1 << (TB - 4)) void k_pass(Cx<R>* __restrict__ stat
 // queue order and only wait on earlier ones, which running workgroups hold,
 // so the queue always drains; the wait is still bounded, and a wait that
 // runs out sets *err instead of spinning on.
-template <typename R, int TB, bool WIDE>
-__global__ __launch_bounds__(1 << (TB - 4)) void k_batch(Cx<R>* __restrict__ states,
+template <typename R, int TB, bool WIDE, int NB>
+__global__ __launch_bounds__(1 << (TB - NB)) void k_batch(Cx<R>* __restrict__ states,
                                                          const uint64_t* __restrict__ circ_off,
                                                          const BatchTask* __restrict__ tasks, uint32_t ntasks,
                                                          uint32_t* __restrict__ next, uint32_t* __restrict__ done,
@@ -619,8 +660,8 @@ __global__ __launch_bounds__(1 << (TB - 4)) void k_batch(Cx<R>* __restrict__ sta
             __syncthreads();
             __threadfence();
         }
-        tile_pass<R, TB, WIDE>(states + circ_off[task.circuit], passes[task.pass], task.tile, tile, micro, ops, coef,
-                               raw);
+        tile_pass<R, TB, WIDE, NB>(states + circ_off[task.circuit], passes[task.pass], task.tile, tile, micro, ops,
+                                   coef, raw);
         // every wave's stores of this tile are out before thread 0 counts it,
         // and every wave has read s_task before thread 0 overwrites it
         __threadfence();
@@ -904,48 +945,67 @@ inline unsigned grid_for(uint64_t n)
     return (unsigned)(b < 65536 ? (b ? b : 1) : 65536);
 }
 
-template <typename R, int TB>
+template <typename R, int TB, int NB>
 int pass_t(void* state, int nq_int, const PassArgs& pa, const DevPlan& dp, hipStream_t s)
 {
     const unsigned grid = 1u << (nq_int - TB);
     if (pa.has_wide)
-        hipLaunchKernelGGL((k_pass<R, TB, true>), dim3(grid), dim3(1 << (TB - 4)), 0, s, (Cx<R>*)state, pa, dp.micro,
-                           dp.ops, (const Cx<R>*)dp.coef, dp.raw);
+        hipLaunchKernelGGL((k_pass<R, TB, true, NB>), dim3(grid), dim3(1 << (TB - NB)), 0, s, (Cx<R>*)state, pa,
+                           dp.micro, dp.ops, (const Cx<R>*)dp.coef, dp.raw);
     else
-        hipLaunchKernelGGL((k_pass<R, TB, false>), dim3(grid), dim3(1 << (TB - 4)), 0, s, (Cx<R>*)state, pa, dp.micro,
-                           dp.ops, (const Cx<R>*)dp.coef, dp.raw);
+        hipLaunchKernelGGL((k_pass<R, TB, false, NB>), dim3(grid), dim3(1 << (TB - NB)), 0, s, (Cx<R>*)state, pa,
+                           dp.micro, dp.ops, (const Cx<R>*)dp.coef, dp.raw);
     return (int)hipGetLastError();
+}
+
+// complex128 runs 3- or 4-bit micro-steps, complex64 4-bit ones.
+template <typename R, int TB>
+int pass_nb(void* state, int nq_int, const PassArgs& pa, const DevPlan& dp, hipStream_t s)
+{
+    if constexpr (sizeof(R) == 8)
+        if (pa.micro_bits == 3) return pass_t<R, TB, 3>(state, nq_int, pa, dp, s);
+    if (pa.micro_bits != 4) return (int)hipErrorInvalidValue;
+    return pass_t<R, TB, 4>(state, nq_int, pa, dp, s);
 }
 
 template <typename R>
 int pass_r(int tile_bits, void* state, int nq_int, const PassArgs& pa, const DevPlan& dp, hipStream_t s)
 {
     switch (tile_bits) {
-    case 10: return pass_t<R, 10>(state, nq_int, pa, dp, s);
-    case 11: return pass_t<R, 11>(state, nq_int, pa, dp, s);
-    case 12: return pass_t<R, 12>(state, nq_int, pa, dp, s);
-    case 13: return pass_t<R, 13>(state, nq_int, pa, dp, s);
+    case 10: return pass_nb<R, 10>(state, nq_int, pa, dp, s);
+    case 11: return pass_nb<R, 11>(state, nq_int, pa, dp, s);
+    case 12: return pass_nb<R, 12>(state, nq_int, pa, dp, s);
+    case 13: return pass_nb<R, 13>(state, nq_int, pa, dp, s);
     default: return (int)hipErrorInvalidValue;
     }
 }
 
-template <typename R, int TB, bool WIDE>
+template <typename R, int TB, bool WIDE, int NB>
 int batch_t(const BatchArgs& a, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_batch<R, TB, WIDE>), dim3(a.grid), dim3(1 << (TB - 4)), 0, s, (Cx<R>*)a.states, a.circ_off,
-                       a.tasks, a.ntasks, a.next, a.done, a.passes, a.dp.micro, a.dp.ops, (const Cx<R>*)a.dp.coef,
-                       a.dp.raw, a.err);
+    hipLaunchKernelGGL((k_batch<R, TB, WIDE, NB>), dim3(a.grid), dim3(1 << (TB - NB)), 0, s, (Cx<R>*)a.states,
+                       a.circ_off, a.tasks, a.ntasks, a.next, a.done, a.passes, a.dp.micro, a.dp.ops,
+                       (const Cx<R>*)a.dp.coef, a.dp.raw, a.err);
     return (int)hipGetLastError();
+}
+
+template <typename R, int TB, bool WIDE>
+int batch_nb(const BatchArgs& a, hipStream_t s)
+{
+    if constexpr (sizeof(R) == 8)
+        if (a.micro_bits == 3) return batch_t<R, TB, WIDE, 3>(a, s);
+    if (a.micro_bits != 4) return (int)hipErrorInvalidValue;
+    return batch_t<R, TB, WIDE, 4>(a, s);
 }
 
 template <typename R, bool WIDE>
 int batch_w(const BatchArgs& a, hipStream_t s)
 {
     switch (a.tile_bits) {
-    case 10: return batch_t<R, 10, WIDE>(a, s);
-    case 11: return batch_t<R, 11, WIDE>(a, s);
-    case 12: return batch_t<R, 12, WIDE>(a, s);
-    case 13: return batch_t<R, 13, WIDE>(a, s);
+    case 10: return batch_nb<R, 10, WIDE>(a, s);
+    case 11: return batch_nb<R, 11, WIDE>(a, s);
+    case 12: return batch_nb<R, 12, WIDE>(a, s);
+    case 13: return batch_nb<R, 13, WIDE>(a, s);
     default: return (int)hipErrorInvalidValue;
     }
 }

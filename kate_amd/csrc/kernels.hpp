@@ -44,6 +44,7 @@ struct BatchArgs {
     const PassArgs* passes;
     DevPlan dp;
     int tile_bits;
+    int micro_bits;  // amplitudes per thread = 2^micro_bits
     bool wide;
     unsigned grid;
 };

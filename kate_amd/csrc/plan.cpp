@@ -81,7 +81,7 @@ int sweep_micro(const std::vector<uint64_t>& masks, const std::vector<uint8_t>& 
 }
 
 uint64_t choose_micro_set(const std::vector<uint64_t>& masks, const std::vector<uint8_t>& ok,
-                          const std::vector<uint8_t>& lazy)
+                          const std::vector<uint8_t>& lazy, int width)
 {
     uint64_t set = 0;
     for (;;) {
@@ -92,7 +92,7 @@ uint64_t choose_micro_set(const std::vector<uint64_t>& masks, const std::vector<
         std::vector<uint64_t> tried;
         for (int f : frontier) {
             const uint64_t add = masks[(size_t)f] & ~set;
-            if (popc(set | add) > 4) continue;
+            if (popc(set | add) > width) continue;
             if (std::find(tried.begin(), tried.end(), add) != tried.end()) continue;
             tried.push_back(add);
             const int gain = sweep_micro(masks, ok, lazy, set | add, nullptr, nullptr, nullptr) - now;
@@ -177,14 +177,65 @@ struct Builder {
     }
 
     // Encodes gate g as a register op on the micro bits `bits` (ascending).
+    // LDS byte offset of layout column `col`: the kernels' XOR swizzle
+    // (low W bits ^= next W bits, W = 4 for 16-byte and 5 for 8-byte
+    // amplitudes) times the amplitude size.
+    uint32_t lds_bytes(uint32_t col) const
+    {
+        const uint32_t w = opt.amp_bytes == 16 ? 4u : 5u;
+        return (col ^ ((col >> w) & ((1u << w) - 1u))) * (uint32_t)opt.amp_bytes;
+    }
+
+    // Order of the non-register tile bits of a register step over the thread
+    // index.  A bank row is 256 bytes (16 complex128 or 32 complex64 slots);
+    // lanes 0..g-1 of a wave (g = slots per row) read one row's worth of
+    // addresses per LDS cycle.  Their addresses differ by XORs of the first
+    // log2(g) thread-bit columns, so choosing columns whose slot indices
+    // (swizzled index mod g) are linearly independent over GF(2) makes those
+    // lanes hit distinct slots.  The remaining bits follow in ascending order.
+    std::vector<int> lane_order(const std::vector<int>& rest, const uint16_t* cols) const
+    {
+        const int lg = opt.amp_bytes == 16 ? 4 : 5;
+        const uint32_t mask = (1u << lg) - 1u;
+        std::vector<int> order, left;
+        uint32_t basis[8] = {0};  // basis[i]: vector with leading bit i
+        auto reduce = [&](uint32_t v) {
+            for (int i = lg - 1; i >= 0; i--)
+                if (((v >> i) & 1u) && basis[i]) v ^= basis[i];
+            return v;
+        };
+        for (int b : rest) {
+            const uint32_t slot = (lds_bytes(cols[b]) / (uint32_t)opt.amp_bytes) & mask;
+            const uint32_t r = (int)order.size() < lg ? reduce(slot) : 0u;
+            if (r) {
+                int hi = 31 - __builtin_clz(r);
+                basis[hi] = r;
+                order.push_back(b);
+            } else {
+                left.push_back(b);
+            }
+        }
+        order.insert(order.end(), left.begin(), left.end());
+        return order;
+    }
+
+    void fill_store_bytes(PassArgs& pa) const
+    {
+        for (int b = 0; b < 16; b++) pa.store_byte[b] = lds_bytes(pa.store_cols[b]);
+    }
+
+    int micro_bit(int tile_bit, const uint8_t* bits) const
+    {
+        for (int j = 0; j < opt.micro_bits; j++)
+            if (bits[j] == tile_bit) return j;
+        return -1;
+    }
+
     std::string add_reg_op(const PlanGate& g, const int* tile_of, const uint8_t* bits)
     {
         int mi[16];
         for (int i = 0; i < g.k; i++) {
-            const int tb = tile_of[g.q[i]];
-            mi[i] = -1;
-            for (int j = 0; j < 4; j++)
-                if (bits[j] == tb) mi[i] = j;
+            mi[i] = micro_bit(tile_of[g.q[i]], bits);
             if (mi[i] < 0) return "internal: gate qubit outside its micro-step";
         }
         MicroOp op{};
@@ -367,12 +418,6 @@ struct Builder {
         double m[8] = {0};  // 2x2 complex, row-major
     };
 
-    static int micro_bit(int tile_bit, const uint8_t* bits)
-    {
-        for (int j = 0; j < 4; j++)
-            if (bits[j] == tile_bit) return j;
-        return -1;
-    }
 
     void hold(Held& h, const PlanGate& g)
     {
@@ -447,6 +492,7 @@ struct Builder {
     std::string pack_micro(const std::vector<int>& gl, const int* tile_of, PassArgs& pa)
     {
         const int t = plan.tile_bits;
+        const int nb = opt.micro_bits;
         std::vector<int> rem = gl;
         pa.micro_begin = (uint32_t)plan.micro.size();
         uint16_t cols[16];
@@ -466,7 +512,7 @@ struct Builder {
                 uint64_t m = 0;
                 for (int j = 0; j < g.k; j++) m |= 1ull << tile_of[g.q[j]];
                 tm[i] = m;
-                ok[i] = g.k <= 4 && !(g.id == 0 && opt.mfma && g.k == 4);
+                ok[i] = g.k <= nb && !(g.id == 0 && opt.mfma && g.k == 4);
                 lazy[i] = g.id == 3;
             }
             if (opt.mfma_fuse) {
@@ -508,7 +554,7 @@ struct Builder {
                 rem.swap(next);
                 continue;
             }
-            const uint64_t set = choose_micro_set(tm, ok, lazy);
+            const uint64_t set = choose_micro_set(tm, ok, lazy, nb);
             std::vector<int> acc, late;
             sweep_micro(tm, ok, lazy, set, nullptr, &acc, &late);
             MicroStep ms{};
@@ -546,9 +592,9 @@ struct Builder {
                 rem.erase(rem.begin());
                 continue;
             }
-            const uint64_t full = pad_set(set, 4, t);
+            const uint64_t full = pad_set(set, nb, t);
             ms.kind = SK_REG;
-            ms.nbits = 4;
+            ms.nbits = (uint8_t)nb;
             int nb = 0;
             for (int b = 0; b < t; b++)
                 if ((full >> b) & 1ull) ms.bits[nb++] = (uint8_t)b;
@@ -572,16 +618,27 @@ struct Builder {
                     std::string e = add_reg_op(g, tile_of, ms.bits);
                     if (!e.empty()) return e;
                 }
-                flush(held, 15);
+                flush(held, (1 << nb) - 1);
             }
             ms.op_count = (uint32_t)(plan.ops.size() - ms.op_begin);
+            if (ms.op_count) {
+                ms.op0_code = plan.ops[ms.op_begin].code;
+                ms.coef = plan.ops[ms.op_begin].coef;
+            }
             {
-                int ti = 0;
+                std::vector<int> rest;
                 for (int b = 0; b < t; b++) {
-                    const bool reg = b == ms.bits[0] || b == ms.bits[1] || b == ms.bits[2] || b == ms.bits[3];
-                    if (!reg && ti < 12) ms.tcols[ti++] = ms.cols[b];
+                    bool reg = false;
+                    for (int k = 0; k < nb; k++) reg = reg || b == ms.bits[k];
+                    if (!reg) rest.push_back(b);
                 }
-                for (int k = 0; k < 4; k++) ms.pcols[k] = ms.cols[ms.bits[k]];
+                const std::vector<int> order = lane_order(rest, ms.cols);
+                int ti = 0;
+                for (int b : order)
+                    if (ti < 12) ms.tcols[ti++] = ms.cols[b];
+                for (int k = 0; k < nb; k++) ms.pcols[k] = ms.cols[ms.bits[k]];
+                for (int i = 0; i < 12; i++) ms.tbyte[i] = lds_bytes(ms.tcols[i]);
+                for (int k = 0; k < 4; k++) ms.pbyte[k] = lds_bytes(ms.pcols[k]);
             }
             plan.micro.push_back(ms);
             for (int a : late) layout_cnot(prog.gates[(size_t)rem[(size_t)a]]);
@@ -600,6 +657,7 @@ struct Builder {
             rem.swap(next);
         }
         std::copy(cols, cols + 16, pa.store_cols);
+        fill_store_bytes(pa);
         pa.micro_count = (uint32_t)(plan.micro.size() - pa.micro_begin);
         pa.has_wide = 0;
         for (uint32_t m = pa.micro_begin; m < plan.micro.size(); m++)
@@ -648,6 +706,7 @@ struct Builder {
             sweep(masks, ok, set, universe, nullptr, &acc);
             if (acc.empty()) return "internal: empty pass";
             PassArgs pa{};
+            pa.micro_bits = (uint8_t)opt.micro_bits;
             int tile_of[kMaxQubits];
             positions(set, pa, tile_of);
             std::vector<int> gl;
@@ -691,6 +750,7 @@ struct Builder {
             if (end == p) return "swap pair does not fit a tile";
             set = pad_set(set, t, n);
             PassArgs pa{};
+            pa.micro_bits = (uint8_t)opt.micro_bits;
             int tile_of[kMaxQubits];
             positions(set, pa, tile_of);
             int where[kMaxQubits];
@@ -700,6 +760,7 @@ struct Builder {
             pa.micro_begin = (uint32_t)plan.micro.size();
             pa.micro_count = 0;
             for (int b = 0; b < 16; b++) pa.store_cols[b] = (uint16_t)(1u << b);
+            fill_store_bytes(pa);
             plan.passes.push_back(pa);
             plan.pass_is_remap.push_back(1);
             plan.remap_passes++;
@@ -717,6 +778,9 @@ std::string build_plan(const Program& prog, const PlanOptions& opt, Plan& plan)
     plan.nq_int = std::max(prog.nqubits, opt.min_qubits);
     if (plan.nq_int > kMaxQubits) return "too many qubits";
     plan.tile_bits = std::min(opt.tile_bits, plan.nq_int);
+    if (opt.micro_bits != 3 && opt.micro_bits != 4) return "micro-steps have 3 or 4 bits";
+    if (opt.amp_bytes != 16 && opt.amp_bytes != 8) return "amplitudes have 8 or 16 bytes";
+    if (opt.micro_bits != 4 && (opt.mfma || opt.mfma_fuse)) return "matrix-core steps need 4-bit micro-steps";
     Builder b{prog, opt, plan};
     std::vector<int> seg;
     size_t grp = 0;
@@ -768,7 +832,7 @@ std::string Plan::describe(bool verbose) const
         for (uint32_t m = 0; m < p.micro_count; m++) {
             const MicroStep& ms = micro[p.micro_begin + m];
             os << "    step " << m << " " << kinds[ms.kind] << " bits=[";
-            const int nb = ms.kind == SK_REG ? 4 : ms.nbits;
+            const int nb = ms.nbits;
             for (int b = 0; b < nb; b++) os << (b ? "," : "") << (int)ms.bits[b];
             os << "]";
             for (uint32_t o = 0; o < ms.op_count; o++) {
@@ -777,8 +841,9 @@ std::string Plan::describe(bool verbose) const
             }
             if (ms.kind == SK_REG) {
                 os << " tcols=";
-                for (int i = 0; i + 4 < p.tile_bits; i++) os << (i ? "," : "") << ms.tcols[i];
-                os << " pcols=" << ms.pcols[0] << "," << ms.pcols[1] << "," << ms.pcols[2] << "," << ms.pcols[3];
+                for (int i = 0; i + ms.nbits < p.tile_bits; i++) os << (i ? "," : "") << ms.tcols[i];
+                os << " pcols=";
+                for (int k = 0; k < 4; k++) os << (k ? "," : "") << ms.pcols[k];
             }
             os << "\n";
         }

@@ -36,10 +36,11 @@ struct MicroOp {
 struct MicroStep {
     uint8_t kind;
     uint8_t nbits;
-    uint8_t pad[2];
+    uint16_t op0_code;  // SK_REG: code of ops[op_begin], read with the step (no dependent load)
     uint32_t op_begin;
     uint32_t op_count;
-    uint32_t coef;     // SK_GENERIC: matrix offset (complex elements); SK_MFMA*: operand offset in Plan::raw
+    uint32_t coef;     // SK_GENERIC: matrix offset (complex elements); SK_MFMA*: operand offset in
+                       // Plan::raw; SK_REG: coefficient offset of ops[op_begin]
     uint8_t bits[16];  // SK_REG: ascending tile bits; SK_GENERIC/SK_MFMA: tile bit of matrix bit i
     // Layout of the LDS tile when the step runs: the amplitude of tile index
     // y sits at LDS index XOR_{b in y} cols[b] (linear over GF(2); CNOTs are
@@ -53,6 +54,11 @@ struct MicroStep {
     // SK_MFMA: layout columns of matrix bit i (gcols) and, in tcols, of the
     // non-gate tile bits in ascending order (the columns of the contraction).
     uint16_t gcols[8];
+    // SK_REG: tcols and pcols as LDS byte offsets after the tile's XOR
+    // swizzle (the swizzle is linear over GF(2), so the byte offset of a
+    // register amplitude is the XOR of these).
+    uint32_t tbyte[12];
+    uint32_t pbyte[4];
 };
 
 constexpr int kMaxQubits = 40;
@@ -67,11 +73,12 @@ struct PassArgs {
     uint8_t load_pos[16];
     uint8_t store_pos[16];
     uint8_t rest_pos[kMaxQubits];
-    uint8_t has_wide;  // some micro-step is not SK_REG (selects the kernel variant)
-    uint8_t pad;
+    uint8_t has_wide;    // some micro-step is not SK_REG (selects the kernel variant)
+    uint8_t micro_bits;  // bits per register micro-step (3 or 4): 2^micro_bits amplitudes per thread
     uint32_t micro_begin;
     uint32_t micro_count;
     uint16_t store_cols[16];  // tile layout at the end of the pass (see MicroStep::cols)
+    uint32_t store_byte[16];  // the same as swizzled LDS byte offsets
 };
 
 struct PlanGate {
@@ -104,6 +111,8 @@ struct PlanOptions {
     bool mfma = false;       // SK_MFMA for dense gates of 4..5 qubits (complex64 only)
     bool mfma_fuse = false;  // also compose the gates of 5-qubit windows into SK_MFMA steps
     bool mfma_bf16 = false;  // matrix-core steps use split-bf16 products (SK_MFMA_BF16)
+    int micro_bits = 4;      // register micro-steps span 3 or 4 tile bits (8 or 16 amplitudes per thread)
+    int amp_bytes = 16;      // 16 (complex128) or 8 (complex64): LDS byte offsets in the plan
 };
 
 struct Plan {

@@ -349,6 +349,7 @@ int kate_server_create(const kate_server_options* options, kate_server** out)
     ko.tile_qubits = o.tile_qubits;
     ko.low_qubits = o.low_qubits;
     ko.use_mfma = o.use_mfma;
+    ko.micro_bits = o.micro_bits;
     ko.device = o.device;
     kate::PlanOptions po;
     std::string err = kate::plan_options(ko, po);

@@ -310,12 +310,18 @@ std::string plan_options(const kate_options& opt, kate::PlanOptions& po)
     if (opt.dtype != KATE_C128 && opt.dtype != KATE_C64) return "dtype must be KATE_C128 or KATE_C64";
     po.tile_bits = opt.tile_qubits ? opt.tile_qubits : 12;
     po.low_qubits = opt.low_qubits ? opt.low_qubits : (opt.dtype == KATE_C128 ? 3 : 4);
-    po.min_qubits = po.tile_bits;
+    // the reductions read whole 4096-amplitude blocks
+    po.min_qubits = std::max(po.tile_bits, 12);
     if (opt.use_mfma && opt.dtype != KATE_C64) return "use_mfma needs KATE_C64 (the FP32 matrix-core path)";
     if (opt.use_mfma < 0 || opt.use_mfma > 3) return "use_mfma must be 0..3";
     po.mfma = opt.use_mfma >= 1;
     po.mfma_fuse = opt.use_mfma >= 2;
     po.mfma_bf16 = opt.use_mfma == 3;
+    po.micro_bits = opt.micro_bits ? opt.micro_bits : 4;
+    po.amp_bytes = opt.dtype == KATE_C128 ? 16 : 8;
+    if (po.micro_bits != 3 && po.micro_bits != 4) return "micro_bits must be 3 or 4";
+    if (po.micro_bits == 3 && opt.dtype != KATE_C128) return "micro_bits 3 needs KATE_C128";
+    if (po.micro_bits == 3 && opt.use_mfma) return "micro_bits 3 excludes use_mfma";
     if (po.tile_bits < 10 || po.tile_bits > 13) return "tile_qubits must be in 10..13";
     if (po.low_qubits < 0 || po.low_qubits > po.tile_bits) return "low_qubits out of range";
     return "";

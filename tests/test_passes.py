@@ -2,6 +2,8 @@
 through the C ABI) against Kate's own passes: golden fixtures, and Kate's CPU
 build live where it is present.  Bit-exact: same gate order, same fused
 matrices, same groups, instructions and permutations."""
+import re
+
 import numpy as np
 import pytest
 
@@ -125,3 +127,36 @@ def test_plan_remaps_swap_instructions():
     nswap = sum(1 for k, _ in c.instructions() if k == 0)
     passes, remaps, _ = c.plan("c128")
     assert nswap > 0 and remaps >= nswap
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_plan_three_bit_micro_steps(seed, monkeypatch):
+    """complex128 plans with 8 amplitudes per thread: every register step
+    spans 3 tile bits, and 4-qubit dense gates leave the register path."""
+    monkeypatch.setenv("KATE_DESCRIBE_OPS", "1")
+    n = 14
+    gates = W.random_mixed(n, 120, 30 + seed, max_dense=4) + W.layered(n, 3, seed)
+    _, _, text = K.Circuit(n, gates).plan("c128", micro_bits=3)
+    reg = [ln for ln in text.splitlines() if " reg bits=[" in ln]
+    assert reg, text[:400]
+    for ln in reg:
+        bits = ln.split("bits=[")[1].split("]")[0].split(",")
+        assert len(bits) == 3, ln
+        assert "pcols=" in ln and ln.rstrip().endswith(",0")  # no fourth register column
+        # every op addresses micro bits 0..2 only
+        for kind, code in re.findall(r" (\w+):(\d+)", ln.split(" tcols=")[0]):
+            c = int(code)
+            idx = {"dense": [], "kron": [], "h": [c], "phase": [c], "cnot": [c // 4, c % 4],
+                   "crk": [c // 4, c % 4], "toff": [c // 16, (c // 4) % 4, c % 4]}[kind]
+            assert all(i < 3 for i in idx) and (kind not in ("dense", "kron") or c < 8), ln
+    if any(len(g.qubits) == 4 and g.id == W.DENSE for g in gates):
+        assert " generic bits=[" in text
+
+
+def test_plan_micro_bits_validation():
+    c = K.Circuit(12, W.layered(12, 2, 1))
+    with pytest.raises(RuntimeError, match="micro_bits 3 needs KATE_C128"):
+        c.plan("c64", micro_bits=3)
+    with pytest.raises(RuntimeError, match="micro_bits must be 3 or 4"):
+        c.plan("c128", micro_bits=5)
+    assert c.plan("c128", micro_bits=3)[0] >= 1
