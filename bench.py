@@ -39,6 +39,35 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+FP64_VALU_PEAK_TFLOPS = 78.6
+
+
+def plan_flops(circuit, dtype, n_int):
+    """Algorithmic flops of the device plan: 8 * 2^n * 2^k real flops per
+    dense k-qubit op as planned (composed 1-qubit runs count once; a
+    Kronecker op of m one-qubit gates counts m dense 1-qubit gates); sparse
+    ops (H, CNOT, CRk, Toffoli, phases) count 0."""
+    import re
+    os.environ["KATE_DESCRIBE_OPS"] = "1"
+    try:
+        text = circuit.plan(dtype)[2]
+    finally:
+        os.environ.pop("KATE_DESCRIBE_OPS", None)
+    flops = 0.0
+    for ln in text.splitlines():
+        if " reg bits=" not in ln and " generic bits=" not in ln and " mfma" not in ln:
+            continue
+        head = ln.split(" tcols=")[0]
+        for kind, code in re.findall(r" (dense|kron):(\d+)", head):
+            m = int(code)
+            k = bin(m).count("1")
+            flops += 8.0 * (1 << n_int) * (2 * k if kind == "kron" else (1 << k))
+        if " generic bits=[" in ln or " mfma" in ln:
+            k = len(ln.split("bits=[")[1].split("]")[0].split(","))
+            flops += 8.0 * (1 << n_int) * (1 << k)
+    return flops
+
+
 def time_executes(sim, steps, warmup):
     for _ in range(warmup):
         sim.execute()
@@ -114,6 +143,10 @@ def main():
     log(f"fp64: {ms64:.2f} ms/circuit, {len(pass_ms)} passes, mean pass {mean_pass:.3f} ms, "
         f"{achieved:.0f} GB/s per pass")
 
+    flops = plan_flops(c, "c128", max(n, 12))
+    tflops = flops / (float(pass_ms.sum()) * 1e-3) / 1e12
+    log(f"fp64 VALU: {flops / 1e12:.2f} TFLOP planned, {tflops:.1f} TFLOP/s over the passes")
+
     out = {
         "metric": METRIC, "value": round(ms64, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms64, 3), "higher_is_better": False, "scaling": "weak",
@@ -125,7 +158,11 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": round(bpp * TRAFFIC_RATIO_30Q_LAYERED) if (n, depth) == (30, 20) else None,
                      "kernel": "k_pass<double,12>", "algorithmic_bytes_per_launch": bpp,
-                     "mean_launch_ms": round(mean_pass, 4), "launches": int(len(gate_ms))},
+                     "mean_launch_ms": round(mean_pass, 4), "launches": int(len(gate_ms)),
+                     "compute": {"bound": "fp64 valu", "achieved": round(tflops, 2),
+                                 "peak": FP64_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                 "frac": round(tflops / FP64_VALU_PEAK_TFLOPS, 4),
+                                 "algorithmic_flops": flops}},
         "passes": int(len(pass_ms)), "init_ms": round(t["init_ms"], 3), "compute_ms": round(t["compute_ms"], 3),
         "measure_ms": round(t["measure_ms"], 3), "circuit_hbm_frac": round(circuit_gbs / HBM_PEAK_GBS, 4),
     }
