@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cstddef>
+#include <cstdlib>
 
 #include "kernels.hpp"
 
@@ -491,43 +492,15 @@ __device__ __forceinline__ uint32_t reg_offset(int s, const uint32_t (&o)[4])
     return x;
 }
 
-// One tile of one pass: load, micro-steps, store.  `blk` is the tile index
-// (its bits are the non-tile qubits rest_pos[]).  Each of the 2^(TB-NB)
-// threads holds N = 2^NB amplitudes; the register bits are tile bits
-// TB-NB..TB-1 when loading and storing.
+// The micro-steps of one pass on the LDS tile (each thread holds 2^NB
+// amplitudes of a step in registers); one barrier after each step.
 template <typename R, int TB, bool WIDE, int NB>
-__device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassArgs& pa, uint64_t blk, Cx<R>* tile,
-                                          const MicroStep* __restrict__ micro, const MicroOp* __restrict__ ops,
-                                          const Cx<R>* __restrict__ coef, const uint32_t* __restrict__ raw)
+__device__ __forceinline__ void run_steps(const PassArgs& pa, Cx<R>* tile, const MicroStep* __restrict__ micro,
+                                          const MicroOp* __restrict__ ops, const Cx<R>* __restrict__ coef,
+                                          const uint32_t* __restrict__ raw)
 {
-    constexpr int NT = 1 << (TB - NB);
     constexpr int N = 1 << NB;
     const uint32_t tid = threadIdx.x;
-
-    const uint64_t base = rest_base(pa, blk);
-
-    {   // load
-        uint64_t tb = base;
-#pragma unroll
-        for (int b = 0; b < TB - NB; b++) tb |= (uint64_t)((tid >> b) & 1u) << pa.load_pos[b];
-        uint64_t m[NB];
-#pragma unroll
-        for (int k = 0; k < NB; k++) m[k] = 1ull << pa.load_pos[TB - NB + k];
-        Cx<R> r[N];
-#pragma unroll
-        for (int i = 0; i < N; i++) {
-            uint64_t off = tb;
-#pragma unroll
-            for (int k = 0; k < NB; k++)
-                if ((i >> k) & 1) off += m[k];
-            r[i] = state[off];
-        }
-        const uint32_t tsw = swz<R>(tid) * (uint32_t)sizeof(Cx<R>);
-#pragma unroll
-        for (int i = 0; i < N; i++) lds_at(tile, tsw ^ (swz<R>((uint32_t)i * NT) * (uint32_t)sizeof(Cx<R>))) = r[i];
-    }
-    __syncthreads();
-
     for (uint32_t mi = 0; mi < pa.micro_count; mi++) {
         const MicroStep* __restrict__ msp = micro + pa.micro_begin + mi;
         if ((reinterpret_cast<const uint32_t*>(msp)[0] & 0xFFu) == SK_REG) {
@@ -568,6 +541,47 @@ __device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassA
         }
         __syncthreads();
     }
+
+}
+
+// One tile of one pass: load, micro-steps, store.  `blk` is the tile index
+// (its bits are the non-tile qubits rest_pos[]).  Each of the 2^(TB-NB)
+// threads holds N = 2^NB amplitudes; the register bits are tile bits
+// TB-NB..TB-1 when loading and storing.
+template <typename R, int TB, bool WIDE, int NB>
+__device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassArgs& pa, uint64_t blk, Cx<R>* tile,
+                                          const MicroStep* __restrict__ micro, const MicroOp* __restrict__ ops,
+                                          const Cx<R>* __restrict__ coef, const uint32_t* __restrict__ raw)
+{
+    constexpr int NT = 1 << (TB - NB);
+    constexpr int N = 1 << NB;
+    const uint32_t tid = threadIdx.x;
+
+    const uint64_t base = rest_base(pa, blk);
+
+    {   // load
+        uint64_t tb = base;
+#pragma unroll
+        for (int b = 0; b < TB - NB; b++) tb |= (uint64_t)((tid >> b) & 1u) << pa.load_pos[b];
+        uint64_t m[NB];
+#pragma unroll
+        for (int k = 0; k < NB; k++) m[k] = 1ull << pa.load_pos[TB - NB + k];
+        Cx<R> r[N];
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            uint64_t off = tb;
+#pragma unroll
+            for (int k = 0; k < NB; k++)
+                if ((i >> k) & 1) off += m[k];
+            r[i] = state[off];
+        }
+        const uint32_t tsw = swz<R>(tid) * (uint32_t)sizeof(Cx<R>);
+#pragma unroll
+        for (int i = 0; i < N; i++) lds_at(tile, tsw ^ (swz<R>((uint32_t)i * NT) * (uint32_t)sizeof(Cx<R>))) = r[i];
+    }
+    __syncthreads();
+
+    run_steps<R, TB, WIDE, NB>(pa, tile, micro, ops, coef, raw);
 
     {   // store, reading the tile through its final layout
         uint64_t tb = base;
