@@ -202,6 +202,7 @@ int kate_batch_create(const kate_program* programs, int32_t count, const kate_op
                 k.tile = t;
                 k.wait_pass = p ? pass_base[c] + p - 1 : 0xFFFFFFFFu;
                 k.wait_count = p ? tiles[c] : 0;
+                k.signal = p + 1 < npass[c] ? 1u : 0u;
                 tasks.push_back(k);
             }
         }

@@ -649,10 +649,22 @@ __global__ __launch_bounds__(1 << (TB - NB)) void k_batch(Cx<R>* __restrict__ st
     // loop head and leaves the other lanes of wave 0 circling an inner loop
     // that re-reads the old task index forever.  The index is also read
     // with readfirstlane so that the exit test is a scalar branch.
+    //
+    // Cross-workgroup visibility costs one agent-scope release (before a
+    // finished task is counted) and one agent-scope acquire (after a wait),
+    // both by thread 0: on this GPU they write back / invalidate the whole
+    // L2 of the XCD (and the CU's L1), which every wave of the workgroup
+    // shares, so one wave's fence covers the workgroup once the barrier
+    // after the tile has drained every wave's stores.  (Fencing in every
+    // thread made the batch four times slower.)
     uint32_t finished = 0xFFFFFFFFu;
     for (;;) {
         if (threadIdx.x == 0) {
-            if (finished != 0xFFFFFFFFu) atomicAdd(done + finished, 1u);
+            if (finished != 0xFFFFFFFFu) {
+                __threadfence();
+                __builtin_amdgcn_s_waitcnt(0);  // the L2 write-back has finished
+                atomicAdd(done + finished, 1u);
+            }
             s_task = atomicAdd(next, 1u);
         }
         __syncthreads();
@@ -670,17 +682,22 @@ __global__ __launch_bounds__(1 << (TB - NB)) void k_batch(Cx<R>* __restrict__ st
                         break;
                     }
                 }
+                __threadfence();
+                __builtin_amdgcn_s_waitcnt(0);  // the cache invalidation has finished
             }
             __syncthreads();
-            __threadfence();
         }
         tile_pass<R, TB, WIDE, NB>(states + circ_off[task.circuit], passes[task.pass], task.tile, tile, micro, ops,
                                    coef, raw);
-        // every wave's stores of this tile are out before thread 0 counts it,
-        // and every wave has read s_task before thread 0 overwrites it
-        __threadfence();
+        // every wave's stores of this tile have completed (each wave drains
+        // its memory counters; a workgroup-scope barrier alone does not wait
+        // for global stores) before thread 0 fences and counts the task, and
+        // every wave has read s_task before thread 0 overwrites it
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        finished = task.pass;
+        // the last pass of a circuit is read only after this launch ends:
+        // no task waits on it, so it is neither fenced nor counted
+        finished = task.signal ? task.pass : 0xFFFFFFFFu;
     }
 }
 

@@ -22,7 +22,8 @@ struct BatchTask {
     uint32_t tile;
     uint32_t wait_pass;   // pass that must be complete first (0xFFFFFFFF: none)
     uint32_t wait_count;  // its tile count
-    uint32_t pad[3];
+    uint32_t signal;      // 1: a later pass of the circuit waits on this one (count it, after a release)
+    uint32_t pad[2];
 };
 
 struct BatchArgs {
