@@ -344,11 +344,8 @@ __device__ __forceinline__ void mfma_dense(Cx<float>* tile, const MicroStep* __r
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ uint32_t bf16_rn(float f)
-{
-    const uint32_t u = __float_as_uint(f);
-    return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
-}
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
 // mfma_dense with split-bf16 products: A = Ahi + Alo (split on the host),
 // B = Bhi + Blo (split here), acc += Ahi Bhi + Ahi Blo + Alo Bhi on
@@ -403,11 +400,12 @@ __device__ __forceinline__ void mfma_bf16(Cx<float>* tile, const MicroStep* __re
                 for (int i = 0; i < K; i++)
                     if (i != 2 && ((g >> i) & 1)) lp ^= gc[i];
                 const Cx<float> x = tile[swz<float>(lp)];
-                const uint32_t hr = bf16_rn(x.re), hm = bf16_rn(x.im);
-                const uint32_t lr = bf16_rn(x.re - __uint_as_float(hr << 16));
-                const uint32_t lm = bf16_rn(x.im - __uint_as_float(hm << 16));
-                hi[jj] = hr | (hm << 16);
-                lo[jj] = lr | (lm << 16);
+                // x = hi + lo in bf16, both rounded to nearest even by the
+                // packed conversion (re in the low half, im in the high half)
+                const uint32_t h = __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{x.re, x.im}, bf16x2));
+                const f32x2 rest{x.re - __uint_as_float(h << 16), x.im - __uint_as_float(h & 0xFFFF0000u)};
+                hi[jj] = h;
+                lo[jj] = __builtin_bit_cast(uint32_t, __builtin_convertvector(rest, bf16x2));
             }
             const bf16x8 bh = __builtin_bit_cast(bf16x8, u32x4{hi[0], hi[1], hi[2], hi[3]});
             const bf16x8 bl = __builtin_bit_cast(bf16x8, u32x4{lo[0], lo[1], lo[2], lo[3]});
@@ -525,13 +523,14 @@ __device__ __forceinline__ void run_steps(const PassArgs& pa, Cx<R>* tile, const
             for (int s = 0; s < N; s++) lds_at(tile, b ^ reg_offset<NB>(s, oc)) = v[s];
         } else if (WIDE) {
             if constexpr (sizeof(R) == 4 && NB == 4) {
+                Cx<float>* tf = reinterpret_cast<Cx<float>*>(tile);
                 if (msp->kind == SK_MFMA) {
                     const float* A = reinterpret_cast<const float*>(raw + msp->coef);
-                    if (msp->nbits == 5) mfma_dense<TB, 5>(reinterpret_cast<Cx<float>*>(tile), msp, A);
-                    else mfma_dense<TB, 4>(reinterpret_cast<Cx<float>*>(tile), msp, A);
+                    if (msp->nbits == 5) mfma_dense<TB, 5>(tf, msp, A);
+                    else mfma_dense<TB, 4>(tf, msp, A);
                 } else if (msp->kind == SK_MFMA_BF16) {
-                    if (msp->nbits == 5) mfma_bf16<TB, 5>(reinterpret_cast<Cx<float>*>(tile), msp, raw + msp->coef);
-                    else mfma_bf16<TB, 4>(reinterpret_cast<Cx<float>*>(tile), msp, raw + msp->coef);
+                    if (msp->nbits == 5) mfma_bf16<TB, 5>(tf, msp, raw + msp->coef);
+                    else mfma_bf16<TB, 4>(tf, msp, raw + msp->coef);
                 } else {
                     generic_dense<R, TB, NB>(tile, msp, coef + msp->coef);
                 }
