@@ -114,6 +114,7 @@ def main():
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-qps", type=float, default=4.0)
+    ap.add_argument("--serve-qps-high", type=float, default=64.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the simulate path it measures is Kate's single-GPU call "
@@ -194,7 +195,7 @@ def main():
         out["batched"] = batched(args.batch_count)
         log(f"batched: {out['batched']}")
     if not args.no_serve:
-        out["serving"] = serving(args.serve_requests, args.serve_qps)
+        out["serving"] = serving(args.serve_requests, args.serve_qps, args.serve_qps_high)
         log(f"serving: {out['serving']}")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(min(n, 22), depth, n, len(gates))
@@ -304,17 +305,24 @@ def batched(count):
             "setup_s": round(time.perf_counter() - t0, 1)}
 
 
-def serving(nreq, qps):
+def serving(nreq, qps, qps_high):
     """Config C5: Poisson arrivals of layered / QFT circuits, n log-uniform
-    in [12, 30], 8 Z strings each; FP64 plain and complex64 + GateMerger(k=5)."""
+    in [12, 30], 8 Z strings each; FP64 plain and complex64 + GateMerger(k=5),
+    at a light load (latency) and at a load above capacity (circuits/s the
+    continuous-batching loop sustains)."""
     from kate_amd.serving import request_mix, serve_poisson
 
     res, zs = {}, {}
     for dtype, fuse in (("c128", 0), ("c64", 5)):
+        label = "fp64_plain" if dtype == "c128" else "fp32_fused"
         reqs = request_mix(nreq, seed=11, fuse=fuse)
         r = serve_poisson(reqs, qps, seed=12, dtype=dtype, streams=4, max_active=32)
         zs[dtype] = r.pop("results")
-        res["fp64_plain" if dtype == "c128" else "fp32_fused"] = r
+        res[label] = r
+        r = serve_poisson(reqs, qps_high, seed=13, dtype=dtype, streams=4, max_active=32)
+        r.pop("results")
+        res[label + "_saturated"] = r
+        log(f"serving {label}: {res[label]}; saturated: {r}")
     res["fp32_fused"]["max_z_err_vs_fp64"] = max(
         float(np.abs(np.asarray(a) - np.asarray(b)).max()) for a, b in zip(zs["c64"], zs["c128"]))
     return res
