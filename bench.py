@@ -114,7 +114,7 @@ def main():
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-qps", type=float, default=4.0)
-    ap.add_argument("--serve-qps-high", type=float, default=64.0)
+    ap.add_argument("--serve-qps-high", type=float, default=1000.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the simulate path it measures is Kate's single-GPU call "
