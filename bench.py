@@ -43,10 +43,12 @@ FP64_VALU_PEAK_TFLOPS = 78.6
 
 
 def plan_flops(circuit, dtype, n_int):
-    """Algorithmic flops of the device plan: 8 * 2^n * 2^k real flops per
-    dense k-qubit op as planned (composed 1-qubit runs count once; a
-    Kronecker op of m one-qubit gates counts m dense 1-qubit gates); sparse
-    ops (H, CNOT, CRk, Toffoli, phases) count 0."""
+    """VALU flops of the device plan as executed, 2 per FMA-class
+    instruction, per amplitude: 8 * 2^k for a dense k-qubit op (composed
+    1-qubit runs count once; a plain Kronecker op of m one-qubit gates counts
+    m dense 1-qubit gates); 8 per bit plus 8 per diagonal table for a
+    Kronecker op in Euler form; sparse ops (H, CNOT, CRk, Toffoli, phases)
+    count 0."""
     import re
     os.environ["KATE_DESCRIBE_OPS"] = "1"
     try:
@@ -58,10 +60,13 @@ def plan_flops(circuit, dtype, n_int):
         if " reg bits=" not in ln and " generic bits=" not in ln and " mfma" not in ln:
             continue
         head = ln.split(" tcols=")[0]
-        for kind, code in re.findall(r" (dense|kron):(\d+)", head):
+        for kind, code in re.findall(r" (dense|kron|ekron):(\d+)", head):
             m = int(code)
-            k = bin(m).count("1")
-            flops += 8.0 * (1 << n_int) * (2 * k if kind == "kron" else (1 << k))
+            k = bin(m & 15).count("1")
+            if kind == "ekron":
+                flops += 8.0 * (1 << n_int) * (k + 1 + (m >> 4))
+            else:
+                flops += 8.0 * (1 << n_int) * (2 * k if kind == "kron" else (1 << k))
         if " generic bits=[" in ln or " mfma" in ln:
             k = len(ln.split("bits=[")[1].split("]")[0].split(","))
             flops += 8.0 * (1 << n_int) * (1 << k)

@@ -94,7 +94,9 @@ typedef struct kate_options {
     int32_t device;       /* HIP device ordinal */
     int32_t micro_bits;   /* amplitudes per thread in register steps = 2^micro_bits:
                              3 (C128 only) or 4; 0 = default per dtype */
-    int32_t reserved;
+    int32_t no_euler;     /* 1 = apply composed 1-qubit gates as dense 2x2 products; 0 = write
+                             products of 2-4 of them as D1 * (RX x ... x RX) * D2 (Euler form,
+                             fewer FMAs; same result to rounding) */
 } kate_options;
 
 /* Wall-clock split of the last execute, as Kate's execute(true) prints it

@@ -188,6 +188,55 @@ __device__ __forceinline__ void op_kron(Cx<R> (&v)[N], const Cx<R>* __restrict__
         if (MASK & 8) op_dense<8>(v, m + 4 * k++);
 }
 
+// Packed index of the bits of s selected by MASK (inverse of dep).
+__host__ __device__ constexpr int ext(int mask, int s)
+{
+    int out = 0, k = 0;
+    for (int b = 0; b < 4; b++)
+        if ((mask >> b) & 1) {
+            if ((s >> b) & 1) out |= 1 << k;
+            k++;
+        }
+    return out;
+}
+
+// Kronecker product of 2x2 gates in Euler form on the micro bits of MASK:
+// gate b is D1_b * RX_b * D2_b with RX_b = [[x, -iy], [-iy, x]] (x, y real),
+// D1_b diagonal and D2_b = diag(1, q_b).  The rotations cost 4 FMA-class
+// instructions per amplitude each (against 8 for a dense 2x2), and the
+// diagonals of all bits are one multiply by a table entry indexed by the
+// amplitude's MASK bits.  Coefficients (complex elements): (x_b, y_b) per
+// bit in ascending order, the D1 table (2^K entries), then, with D2, the D2
+// table (its entry 0 is 1 and is not applied).
+template <int MASK, bool HAS_D2, typename R, int N>
+__device__ __forceinline__ void op_ekron(Cx<R> (&v)[N], const Cx<R>* __restrict__ m)
+{
+    constexpr int K = popc4(MASK);
+    if constexpr (HAS_D2) {
+        const Cx<R>* __restrict__ t2 = m + K + (1 << K);
+#pragma unroll
+        for (int s = 0; s < N; s++)
+            if (ext(MASK, s)) v[s] = cmul(t2[ext(MASK, s)], v[s]);
+    }
+    int k = 0;
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        if (!((MASK >> b) & 1)) continue;
+        const R x = m[k].re, y = m[k].im;
+        k++;
+#pragma unroll
+        for (int s = 0; s < N; s++) {
+            if ((s >> b) & 1) continue;
+            const Cx<R> a = v[s], c = v[s | (1 << b)];
+            v[s] = Cx<R>{x * a.re + y * c.im, x * a.im - y * c.re};
+            v[s | (1 << b)] = Cx<R>{x * c.re + y * a.im, x * c.im - y * a.re};
+        }
+    }
+    const Cx<R>* __restrict__ t1 = m + K;
+#pragma unroll
+    for (int s = 0; s < N; s++) v[s] = cmul(t1[ext(MASK, s)], v[s]);
+}
+
 #define KATE_OPC(kind, var) (((kind) << 8) | (var))
 
 template <typename R, int N>
@@ -199,6 +248,11 @@ __device__ __forceinline__ void apply_op(Cx<R> (&v)[N], uint32_t code, const Cx<
     case KATE_OPC(OK_KRON, M): if constexpr ((M) < N) op_kron<M>(v, cf); break;
         D(1) D(2) D(3) D(4) D(5) D(6) D(7) D(8) D(9) D(10) D(11) D(12) D(13) D(14) D(15)
 #undef D
+#define E(M) \
+    case KATE_OPC(OK_EKRON, M): if constexpr ((M) < N) op_ekron<M, false>(v, cf); break; \
+    case KATE_OPC(OK_EKRON, (M) | 16): if constexpr ((M) < N) op_ekron<M, true>(v, cf); break;
+        E(3) E(5) E(6) E(7) E(9) E(10) E(11) E(12) E(13) E(14) E(15)
+#undef E
 #define P(J) \
     case KATE_OPC(OK_PHASE, J): if constexpr ((1 << (J)) < N) op_phase<J>(v, cf); break; \
     case KATE_OPC(OK_H, J): if constexpr ((1 << (J)) < N) op_h<J>(v); break;
@@ -513,9 +567,12 @@ __device__ __forceinline__ void run_steps(const PassArgs& pa, Cx<R>* tile, const
             for (uint32_t o = 0; o < op_count; o++) {
                 uint32_t code = code0, cofs = coef0;
                 if (o) {
-                    const MicroOp op = ops[op_begin + o];
-                    code = op.code;
-                    cofs = op.coef;
+                    // read as 32-bit words: a 16-bit field would be a vector
+                    // load (no 16-bit scalar loads), whose wait also drains
+                    // every other vector memory operation of the wave
+                    const uint32_t* ow = reinterpret_cast<const uint32_t*>(ops + op_begin + o);
+                    code = ow[0] & 0xFFFFu;
+                    cofs = ow[1];
                 }
                 apply_op<R>(v, code, coef + cofs);
             }
@@ -541,6 +598,40 @@ __device__ __forceinline__ void run_steps(const PassArgs& pa, Cx<R>* tile, const
         __syncthreads();
     }
 
+}
+
+// Store of one tile, reading it through its final layout: the same
+// coalesced runs as the load, optionally to permuted positions (store_pos).
+template <typename R, int TB, int NB>
+__device__ __forceinline__ void store_tile(Cx<R>* __restrict__ state, const PassArgs& pa, uint64_t base,
+                                           Cx<R>* tile)
+{
+    constexpr int N = 1 << NB;
+    const uint32_t tid = threadIdx.x;
+    {
+        uint64_t tb = base;
+        uint32_t lt = 0;
+#pragma unroll
+        for (int b = 0; b < TB - NB; b++) {
+            tb |= (uint64_t)((tid >> b) & 1u) << pa.store_pos[b];
+            if ((tid >> b) & 1u) lt ^= pa.store_byte[b];
+        }
+        uint64_t m[NB];
+        uint32_t l[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < NB; k++) {
+            m[k] = 1ull << pa.store_pos[TB - NB + k];
+            l[k] = pa.store_byte[TB - NB + k];
+        }
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            uint64_t off = tb;
+#pragma unroll
+            for (int k = 0; k < NB; k++)
+                if ((i >> k) & 1) off += m[k];
+            state[off] = lds_at(tile, lt ^ reg_offset<NB>(i, l));
+        }
+    }
 }
 
 // One tile of one pass: load, micro-steps, store.  `blk` is the tile index
@@ -581,31 +672,7 @@ __device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassA
     __syncthreads();
 
     run_steps<R, TB, WIDE, NB>(pa, tile, micro, ops, coef, raw);
-
-    {   // store, reading the tile through its final layout
-        uint64_t tb = base;
-        uint32_t lt = 0;
-#pragma unroll
-        for (int b = 0; b < TB - NB; b++) {
-            tb |= (uint64_t)((tid >> b) & 1u) << pa.store_pos[b];
-            if ((tid >> b) & 1u) lt ^= pa.store_byte[b];
-        }
-        uint64_t m[NB];
-        uint32_t l[4] = {0, 0, 0, 0};
-#pragma unroll
-        for (int k = 0; k < NB; k++) {
-            m[k] = 1ull << pa.store_pos[TB - NB + k];
-            l[k] = pa.store_byte[TB - NB + k];
-        }
-#pragma unroll
-        for (int i = 0; i < N; i++) {
-            uint64_t off = tb;
-#pragma unroll
-            for (int k = 0; k < NB; k++)
-                if ((i >> k) & 1) off += m[k];
-            state[off] = lds_at(tile, lt ^ reg_offset<NB>(i, l));
-        }
-    }
+    store_tile<R, TB, NB>(state, pa, base, tile);
 }
 
 template <typename R, int TB, bool WIDE, int NB>

@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstring>
 #include <sstream>
 
@@ -452,9 +453,89 @@ struct Builder {
         h.count++;
     }
 
+    // Euler form of a 2x2 unitary u (row-major, interleaved):
+    // u = diag(p0, p1) * [[x, -iy], [-iy, x]] * diag(1, q) with x = |u00| and
+    // y real.  q is set to exactly 1 (has_q false) when it differs from 1 by at
+    // most 1e-15, as for RZ * RX.  Fails when a zero entry makes the phases
+    // undefined or when the product differs from u by more than 1e-14 (a
+    // matrix that is not unitary).
+    static bool euler_form(const double* u, double& x, double& y, std::complex<double> (&p)[2],
+                           std::complex<double>& q, bool& has_q)
+    {
+        using cd = std::complex<double>;
+        const cd u00(u[0], u[1]), u01(u[2], u[3]), u10(u[4], u[5]), u11(u[6], u[7]);
+        const cd I(0.0, 1.0);
+        x = std::abs(u00);
+        if (!(x > 1e-150) || !(std::abs(u01) > 1e-150)) return false;
+        p[0] = u00 / x;
+        const cd yq = I * u01 / p[0];  // u01 = -i p0 y q
+        if (std::abs(yq.imag()) <= 1e-15 * std::abs(yq)) {
+            y = yq.real();
+            q = 1.0;
+            has_q = false;
+        } else {
+            y = std::abs(yq);
+            q = yq / y;
+            has_q = true;
+        }
+        p[1] = I * u10 / y;  // u10 = -i p1 y
+        const cd r00 = p[0] * x, r01 = -I * p[0] * y * q, r10 = -I * p[1] * y, r11 = p[1] * x * q;
+        const double err = std::max(std::max(std::abs(r00 - u00), std::abs(r01 - u01)),
+                                    std::max(std::abs(r10 - u10), std::abs(r11 - u11)));
+        return err <= 1e-14;
+    }
+
+    // The held dense gates of the micro bits in `kron` as one Euler-form
+    // Kronecker op (OK_EKRON), when every one has an Euler form and the op
+    // is cheaper than the plain Kronecker op: 4 instructions per amplitude
+    // per bit plus 4 per diagonal table, against 8 per bit.
+    bool emit_ekron(Held* held, int kron)
+    {
+        using cd = std::complex<double>;
+        const int K = __builtin_popcount(kron);
+        if (K < 2) return false;
+        double xs[4], ys[4];
+        cd ps[4][2], qs[4];
+        bool any_q = false;
+        int j = 0;
+        for (int b = 0; b < 4; b++) {
+            if (!((kron >> b) & 1)) continue;
+            bool hq = false;
+            if (!euler_form(held[b].m, xs[j], ys[j], ps[j], qs[j], hq)) return false;
+            any_q = any_q || hq;
+            j++;
+        }
+        if (any_q && K < 3) return false;
+        MicroOp op{};
+        op.coef = (uint32_t)(plan.coef.size() / 2);
+        for (int i = 0; i < K; i++) {
+            const double c[2] = {xs[i], ys[i]};
+            add_coef(c, 1);
+        }
+        for (int e = 0; e < (1 << K); e++) {
+            cd t = 1.0;
+            for (int i = 0; i < K; i++) t *= ps[i][(e >> i) & 1];
+            const double c[2] = {t.real(), t.imag()};
+            add_coef(c, 1);
+        }
+        if (any_q)
+            for (int e = 0; e < (1 << K); e++) {
+                cd t = 1.0;
+                for (int i = 0; i < K; i++)
+                    if ((e >> i) & 1) t *= qs[i];
+                const double c[2] = {t.real(), t.imag()};
+                add_coef(c, 1);
+            }
+        for (int b = 0; b < 4; b++)
+            if ((kron >> b) & 1) held[b] = Held();
+        op.code = (uint16_t)(OK_EKRON << 8 | kron | (any_q ? 16 : 0));
+        plan.ops.push_back(op);
+        return true;
+    }
+
     // Emits the held gates of the micro bits in `mask`: a lone Hadamard keeps
     // Kate's Hadamard arithmetic, a diagonal becomes a phase op, the rest go
-    // out together as one Kronecker op.
+    // out together as one Kronecker op (in Euler form where that is cheaper).
     void flush(Held* held, int mask)
     {
         int kron = 0;
@@ -477,6 +558,7 @@ struct Builder {
             }
         }
         if (!kron) return;
+        if (opt.euler && emit_ekron(held, kron)) return;
         MicroOp op{};
         op.coef = (uint32_t)(plan.coef.size() / 2);
         for (int b = 0; b < 4; b++)
@@ -828,7 +910,7 @@ std::string Plan::describe(bool verbose) const
         os << " micro=" << p.micro_count << " ops=" << nops << "\n";
         if (!verbose) continue;
         static const char* kinds[] = {"reg", "generic", "mfma", "mfma_bf16"};
-        static const char* okinds[] = {"dense", "phase", "h", "cnot", "crk", "toff", "kron"};
+        static const char* okinds[] = {"dense", "phase", "h", "cnot", "crk", "toff", "kron", "ekron"};
         for (uint32_t m = 0; m < p.micro_count; m++) {
             const MicroStep& ms = micro[p.micro_begin + m];
             os << "    step " << m << " " << kinds[ms.kind] << " bits=[";

@@ -18,6 +18,8 @@ enum OpKind : uint16_t {
     OK_CRK = 4,    // variant = a*4 + b, a < b
     OK_TOFF = 5,   // variant = target*16 + c0*4 + c1, c0 < c1
     OK_KRON = 6,   // variant = mask of micro bits; one 2x2 per set bit, ascending, applied in turn
+    OK_EKRON = 7,  // variant = mask of micro bits (2+ bits) | 16 if the right diagonal is present;
+                   // the same product with every 2x2 in Euler form D1 * RX * D2
 };
 
 enum StepKind : uint8_t {
@@ -113,6 +115,7 @@ struct PlanOptions {
     bool mfma_bf16 = false;  // matrix-core steps use split-bf16 products (SK_MFMA_BF16)
     int micro_bits = 4;      // register micro-steps span 3 or 4 tile bits (8 or 16 amplitudes per thread)
     int amp_bytes = 16;      // 16 (complex128) or 8 (complex64): LDS byte offsets in the plan
+    bool euler = true;       // Kronecker ops of 2+ composed 1-qubit gates in Euler form (OK_EKRON)
 };
 
 struct Plan {

@@ -77,6 +77,19 @@ def haar_unitary(rng: np.random.Generator, dim: int) -> np.ndarray:
     return (q * (d / np.abs(d))).astype(np.complex128)
 
 
+def haar_layers(n: int, layers: int, seed: int) -> List[Gate]:
+    """A Haar-random U(2) on every qubit, then the CNOT ladder i -> i+1, per
+    layer: general one-qubit gates (no RX/RZ structure)."""
+    rng = np.random.default_rng(seed)
+    gates: List[Gate] = []
+    for _ in range(layers):
+        for q in range(n):
+            gates.append(Gate(DENSE, (q,), 0, haar_unitary(rng, 2)))
+        for i in range(n - 1):
+            gates.append(Gate(CNOT, (i + 1, i)))
+    return gates
+
+
 def brickwork(n: int, depth: int, seed: int, longrange_every: int = 4, longrange_offset: int = 16) -> List[Gate]:
     """Haar 2-qubit brickwork: even layers pair (2j, 2j+1), odd layers
     (2j+1, 2j+2).  Every ``longrange_every``-th layer (1-based) also applies

@@ -146,9 +146,10 @@ def test_plan_three_bit_micro_steps(seed, monkeypatch):
         # every op addresses micro bits 0..2 only
         for kind, code in re.findall(r" (\w+):(\d+)", ln.split(" tcols=")[0]):
             c = int(code)
-            idx = {"dense": [], "kron": [], "h": [c], "phase": [c], "cnot": [c // 4, c % 4],
+            idx = {"dense": [], "kron": [], "ekron": [], "h": [c], "phase": [c], "cnot": [c // 4, c % 4],
                    "crk": [c // 4, c % 4], "toff": [c // 16, (c // 4) % 4, c % 4]}[kind]
-            assert all(i < 3 for i in idx) and (kind not in ("dense", "kron") or c < 8), ln
+            mask = c & 15 if kind == "ekron" else c
+            assert all(i < 3 for i in idx) and (kind not in ("dense", "kron", "ekron") or mask < 8), ln
     if any(len(g.qubits) == 4 and g.id == W.DENSE for g in gates):
         assert " generic bits=[" in text
 
@@ -160,3 +161,25 @@ def test_plan_micro_bits_validation():
     with pytest.raises(RuntimeError, match="micro_bits must be 3 or 4"):
         c.plan("c128", micro_bits=5)
     assert c.plan("c128", micro_bits=3)[0] >= 1
+
+
+def test_plan_euler_form_ops(monkeypatch):
+    """Kronecker steps of composed 1-qubit gates are planned in Euler form:
+    RX/RZ layers need no right diagonal (variant < 16), Haar 1-qubit layers
+    do (variant | 16); euler=False keeps the dense 2x2 products."""
+    monkeypatch.setenv("KATE_DESCRIBE_OPS", "1")
+    n = 14
+
+    def kinds(gates, **kw):
+        _, _, text = K.Circuit(n, gates).plan("c128", **kw)
+        return re.findall(r" (kron|ekron):(\d+)", text)
+
+    lay = kinds(W.layered(n, 4, 1))
+    assert lay and all(k == "ekron" and int(c) < 16 and bin(int(c)).count("1") >= 2 for k, c in lay)
+    plain = kinds(W.layered(n, 4, 1), euler=False)
+    assert plain and all(k == "kron" for k, _ in plain)
+    assert sorted(int(c) for _, c in plain) == sorted(int(c) for _, c in lay)
+    haar = kinds(W.haar_layers(n, 4, 2))
+    assert any(k == "ekron" and int(c) >= 16 for k, c in haar)
+    # with the right diagonal, a 2-bit Euler op is no cheaper than the plain one
+    assert all(k == "kron" or int(c) < 16 or bin(int(c) & 15).count("1") >= 3 for k, c in haar)
