@@ -168,7 +168,7 @@ def main():
                      "compute": {"bound": "fp64 valu", "achieved": round(tflops, 2),
                                  "peak": FP64_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
                                  "frac": round(tflops / FP64_VALU_PEAK_TFLOPS, 4),
-                                 "algorithmic_flops": flops}},
+                                 "valu_flops": flops}},
         "passes": int(len(pass_ms)), "init_ms": round(t["init_ms"], 3), "compute_ms": round(t["compute_ms"], 3),
         "measure_ms": round(t["measure_ms"], 3), "circuit_hbm_frac": round(circuit_gbs / HBM_PEAK_GBS, 4),
     }

@@ -94,10 +94,20 @@ typedef struct kate_options {
     int32_t device;       /* HIP device ordinal */
     int32_t micro_bits;   /* amplitudes per thread in register steps = 2^micro_bits:
                              3 (C128 only) or 4; 0 = default per dtype */
-    int32_t no_euler;     /* 1 = apply composed 1-qubit gates as dense 2x2 products; 0 = write
-                             products of 2-4 of them as D1 * (RX x ... x RX) * D2 (Euler form,
-                             fewer FMAs; same result to rounding) */
+    int32_t plan_flags;   /* 0 = every plan optimisation; KATE_PLAN_* bits switch one off */
 } kate_options;
+
+/* kate_options.plan_flags */
+enum {
+    /* apply composed 1-qubit gates as dense 2x2 products instead of writing
+       products of 2-4 of them as D1 * (RX x ... x RX) * D2 (Euler form:
+       fewer FMAs, same result to rounding) */
+    KATE_PLAN_NO_EULER = 1,
+    /* fill and drain every LDS tile through LDS instead of loading the
+       first micro-step's amplitudes from HBM into registers and storing the
+       last one's from registers (same arithmetic, bit for bit) */
+    KATE_PLAN_NO_DIRECT = 2
+};
 
 /* Wall-clock split of the last execute, as Kate's execute(true) prints it
  * (HIPcode/simulator.hpp:301-353), plus the pass statistics of the plan. */

@@ -37,7 +37,14 @@ class _GateRec(C.Structure):
 class Options(C.Structure):
     _fields_ = [("dtype", C.c_int32), ("tile_qubits", C.c_int32), ("low_qubits", C.c_int32),
                 ("use_mfma", C.c_int32), ("fold_remaps", C.c_int32), ("device", C.c_int32),
-                ("micro_bits", C.c_int32), ("no_euler", C.c_int32)]
+                ("micro_bits", C.c_int32), ("plan_flags", C.c_int32)]
+
+
+PLAN_NO_EULER, PLAN_NO_DIRECT = 1, 2
+
+
+def _plan_flags(euler: bool, direct: bool) -> int:
+    return (0 if euler else PLAN_NO_EULER) | (0 if direct else PLAN_NO_DIRECT)
 
 
 class Timings(C.Structure):
@@ -272,10 +279,10 @@ class Circuit:
         return out
 
     def plan(self, dtype: str = "c128", tile_qubits: int = 0, low_qubits: int = 0, mfma: int = 0,
-             micro_bits: int = 0, euler: bool = True) -> Tuple[int, int, str]:
+             micro_bits: int = 0, euler: bool = True, direct: bool = True) -> Tuple[int, int, str]:
         """(gate passes, remap passes, text) of the device plan, without a GPU."""
         opt = Options({"c128": KATE_C128, "c64": KATE_C64}[dtype], tile_qubits, low_qubits, mfma, 0, 0, micro_bits,
-                      0 if euler else 1)
+                      _plan_flags(euler, direct))
         p, r = C.c_int32(), C.c_int32()
         buf = C.create_string_buffer(1 << 20)
         _check(lib().kate_circuit_plan(self._h, C.byref(opt), C.byref(p), C.byref(r), buf, 1 << 20), "plan")
@@ -296,11 +303,11 @@ class Simulator:
     """Kate::Simulator on one GPU (HIPcode/simulator.hpp:220-543)."""
 
     def __init__(self, circuit: Circuit, dtype: str = "c128", tile_qubits: int = 0, low_qubits: int = 0,
-                 device: int = 0, mfma: int = 0, micro_bits: int = 0, euler: bool = True):
+                 device: int = 0, mfma: int = 0, micro_bits: int = 0, euler: bool = True, direct: bool = True):
         self.n = circuit.n
         self.dtype = {"c128": KATE_C128, "c64": KATE_C64}[dtype]
         self.amp = np.complex128 if self.dtype == KATE_C128 else np.complex64
-        opt = Options(self.dtype, tile_qubits, low_qubits, mfma, 0, device, micro_bits, 0 if euler else 1)
+        opt = Options(self.dtype, tile_qubits, low_qubits, mfma, 0, device, micro_bits, _plan_flags(euler, direct))
         h = VP()
         _check(lib().kate_sim_create_from_circuit(circuit._h, C.byref(opt), C.byref(h)), "kate_sim_create")
         self._h = h

@@ -546,36 +546,67 @@ __device__ __forceinline__ uint32_t reg_offset(int s, const uint32_t (&o)[4])
 
 // The micro-steps of one pass on the LDS tile (each thread holds 2^NB
 // amplitudes of a step in registers); one barrier after each step.
+// The ops of an SK_REG step on the thread's registers.  The first op comes
+// with the step (no load that waits on the step); the others are read as
+// 32-bit words: a 16-bit field would be a vector load (there are no 16-bit
+// scalar loads), whose wait also drains every other vector memory operation
+// of the wave.
+template <typename R, int N>
+__device__ __forceinline__ void reg_step_ops(Cx<R> (&v)[N], const MicroStep* __restrict__ msp, uint32_t op_begin,
+                                             uint32_t op_count, const MicroOp* __restrict__ ops,
+                                             const Cx<R>* __restrict__ coef)
+{
+    const uint32_t w0 = reinterpret_cast<const uint32_t*>(msp)[0];
+    const uint32_t code0 = w0 >> 16, coef0 = reinterpret_cast<const uint32_t*>(msp)[3];
+    for (uint32_t o = 0; o < op_count; o++) {
+        uint32_t code = code0, cofs = coef0;
+        if (o) {
+            const uint32_t* ow = reinterpret_cast<const uint32_t*>(ops + op_begin + o);
+            code = ow[0] & 0xFFFFu;
+            cofs = ow[1];
+        }
+        apply_op<R>(v, code, coef + cofs);
+    }
+}
+
+// State offset of register amplitude s of this thread for a direct step:
+// the XOR of the columns of the set thread-index and register bits.
+template <int TB, int NB>
+__device__ __forceinline__ uint64_t direct_thread_offset(const uint64_t* __restrict__ g, uint32_t tid)
+{
+    uint64_t off = 0;
+#pragma unroll
+    for (int i = 0; i < TB - NB; i++)
+        if ((tid >> i) & 1u) off ^= g[i];
+    return off;
+}
+
+template <int NB>
+__device__ __forceinline__ uint64_t direct_reg_offset(const uint64_t* __restrict__ g, int s)
+{
+    uint64_t off = 0;
+#pragma unroll
+    for (int k = 0; k < NB; k++)
+        if ((s >> k) & 1) off ^= g[12 + k];
+    return off;
+}
+
 template <typename R, int TB, bool WIDE, int NB>
 __device__ __forceinline__ void run_steps(const PassArgs& pa, Cx<R>* tile, const MicroStep* __restrict__ micro,
                                           const MicroOp* __restrict__ ops, const Cx<R>* __restrict__ coef,
-                                          const uint32_t* __restrict__ raw)
+                                          const uint32_t* __restrict__ raw, uint32_t m_begin, uint32_t m_end)
 {
     constexpr int N = 1 << NB;
     const uint32_t tid = threadIdx.x;
-    for (uint32_t mi = 0; mi < pa.micro_count; mi++) {
+    for (uint32_t mi = m_begin; mi < m_end; mi++) {
         const MicroStep* __restrict__ msp = micro + pa.micro_begin + mi;
         if ((reinterpret_cast<const uint32_t*>(msp)[0] & 0xFFu) == SK_REG) {
             uint32_t op_begin, op_count, oc[4], b;
             step_addressing<TB, NB>(msp, tid, op_begin, op_count, oc, b);
-            const uint32_t w0 = reinterpret_cast<const uint32_t*>(msp)[0];
-            const uint32_t code0 = w0 >> 16, coef0 = reinterpret_cast<const uint32_t*>(msp)[3];
             Cx<R> v[N];
 #pragma unroll
             for (int s = 0; s < N; s++) v[s] = lds_at(tile, b ^ reg_offset<NB>(s, oc));
-            // the first op comes with the step (no load that waits on the step)
-            for (uint32_t o = 0; o < op_count; o++) {
-                uint32_t code = code0, cofs = coef0;
-                if (o) {
-                    // read as 32-bit words: a 16-bit field would be a vector
-                    // load (no 16-bit scalar loads), whose wait also drains
-                    // every other vector memory operation of the wave
-                    const uint32_t* ow = reinterpret_cast<const uint32_t*>(ops + op_begin + o);
-                    code = ow[0] & 0xFFFFu;
-                    cofs = ow[1];
-                }
-                apply_op<R>(v, code, coef + cofs);
-            }
+            reg_step_ops<R, N>(v, msp, op_begin, op_count, ops, coef);
 #pragma unroll
             for (int s = 0; s < N; s++) lds_at(tile, b ^ reg_offset<NB>(s, oc)) = v[s];
         } else if (WIDE) {
@@ -648,8 +679,28 @@ __device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassA
     const uint32_t tid = threadIdx.x;
 
     const uint64_t base = rest_base(pa, blk);
+    uint32_t m_begin = 0, m_end = pa.micro_count;
 
-    {   // load
+    if (pa.fuse_first) {
+        // direct first step: HBM -> registers -> ops -> LDS in the step's layout
+        const MicroStep* __restrict__ msp = micro + pa.micro_begin;
+        uint32_t op_begin, op_count, oc[4], b;
+        step_addressing<TB, NB>(msp, tid, op_begin, op_count, oc, b);
+        const uint64_t gt = base ^ direct_thread_offset<TB, NB>(pa.first_g, tid);
+        Cx<R> v[N];
+#pragma unroll
+        for (int s = 0; s < N; s++) v[s] = state[gt ^ direct_reg_offset<NB>(pa.first_g, s)];
+        reg_step_ops<R, N>(v, msp, op_begin, op_count, ops, coef);
+        m_begin = 1;
+        if (pa.micro_count == 1 && pa.fuse_last) {
+            const uint64_t gs = base ^ direct_thread_offset<TB, NB>(pa.last_g, tid);
+#pragma unroll
+            for (int s = 0; s < N; s++) state[gs ^ direct_reg_offset<NB>(pa.last_g, s)] = v[s];
+            return;
+        }
+#pragma unroll
+        for (int s = 0; s < N; s++) lds_at(tile, b ^ reg_offset<NB>(s, oc)) = v[s];
+    } else {   // load
         uint64_t tb = base;
 #pragma unroll
         for (int b = 0; b < TB - NB; b++) tb |= (uint64_t)((tid >> b) & 1u) << pa.load_pos[b];
@@ -671,8 +722,23 @@ __device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassA
     }
     __syncthreads();
 
-    run_steps<R, TB, WIDE, NB>(pa, tile, micro, ops, coef, raw);
-    store_tile<R, TB, NB>(state, pa, base, tile);
+    if (pa.fuse_last) m_end--;
+    run_steps<R, TB, WIDE, NB>(pa, tile, micro, ops, coef, raw, m_begin, m_end);
+    if (pa.fuse_last) {
+        // direct last step: LDS in the step's layout -> registers -> ops -> HBM
+        const MicroStep* __restrict__ msp = micro + pa.micro_begin + m_end;
+        uint32_t op_begin, op_count, oc[4], b;
+        step_addressing<TB, NB>(msp, tid, op_begin, op_count, oc, b);
+        Cx<R> v[N];
+#pragma unroll
+        for (int s = 0; s < N; s++) v[s] = lds_at(tile, b ^ reg_offset<NB>(s, oc));
+        reg_step_ops<R, N>(v, msp, op_begin, op_count, ops, coef);
+        const uint64_t gs = base ^ direct_thread_offset<TB, NB>(pa.last_g, tid);
+#pragma unroll
+        for (int s = 0; s < N; s++) state[gs ^ direct_reg_offset<NB>(pa.last_g, s)] = v[s];
+    } else {
+        store_tile<R, TB, NB>(state, pa, base, tile);
+    }
 }
 
 template <typename R, int TB, bool WIDE, int NB>

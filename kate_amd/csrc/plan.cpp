@@ -225,6 +225,86 @@ struct Builder {
         for (int b = 0; b < 16; b++) pa.store_byte[b] = lds_bytes(pa.store_cols[b]);
     }
 
+    // Direct first / last micro-step of a gate pass (see PassArgs): the state
+    // offset of each thread-index and register bit of the step.  The first
+    // step reads LDS index P, which the plain fill loads from tile index P
+    // (the tile starts in load order); the last step writes LDS index P,
+    // which the plain drain stores as tile index L^-1(P) for the final
+    // layout L.  Only taken when the low qubits stay on lane bits 0..low-1
+    // in order (coalesced runs of 2^low amplitudes, as in the plain fill and
+    // drain).
+    void fill_direct(PassArgs& pa) const
+    {
+        pa.fuse_first = pa.fuse_last = 0;
+        std::fill(pa.first_g, pa.first_g + 16, 0ull);
+        std::fill(pa.last_g, pa.last_g + 16, 0ull);
+        if (!opt.direct || pa.micro_count == 0) return;
+        const int t = plan.tile_bits, nb = opt.micro_bits, nt = t - nb;
+        const int low = std::min(opt.low_qubits, nt);
+        auto offset = [&](uint32_t y, const uint8_t* pos) {
+            uint64_t g = 0;
+            for (int j = 0; j < t; j++)
+                if ((y >> j) & 1u) g ^= 1ull << pos[j];
+            return g;
+        };
+        auto coalesced = [&](const uint64_t* g) {
+            for (int i = 0; i < low; i++)
+                if (g[i] != (1ull << i)) return false;
+            return true;
+        };
+        uint64_t g[16];
+        const MicroStep& f = plan.micro[pa.micro_begin];
+        if (f.kind == SK_REG) {
+            std::fill(g, g + 16, 0ull);
+            for (int i = 0; i < nt; i++) g[i] = offset(f.tcols[i], pa.load_pos);
+            for (int k = 0; k < nb; k++) g[12 + k] = offset(f.pcols[k], pa.load_pos);
+            if (coalesced(g)) {
+                std::copy(g, g + 16, pa.first_g);
+                pa.fuse_first = 1;
+            }
+        }
+        const MicroStep& l = plan.micro[pa.micro_begin + pa.micro_count - 1];
+        if (l.kind == SK_REG) {
+            // inverse of the final layout, column by column: inv[j] = tile
+            // index stored at LDS index 2^j (Gauss-Jordan over GF(2))
+            uint32_t a[16], inv[16];
+            for (int b = 0; b < t; b++) {
+                a[b] = pa.store_cols[b];
+                inv[b] = 1u << b;
+            }
+            for (int j = 0; j < t; j++) {
+                int piv = -1;
+                for (int b = j; b < t; b++)
+                    if ((a[b] >> j) & 1u) {
+                        piv = b;
+                        break;
+                    }
+                if (piv < 0) return;  // not invertible: keep the plain drain
+                std::swap(a[j], a[piv]);
+                std::swap(inv[j], inv[piv]);
+                for (int b = 0; b < t; b++)
+                    if (b != j && ((a[b] >> j) & 1u)) {
+                        a[b] ^= a[j];
+                        inv[b] ^= inv[j];
+                    }
+            }
+            // now a[j] = 2^j and inv[j] = the tile index whose image is 2^j
+            auto tile_index = [&](uint32_t p) {
+                uint32_t y = 0;
+                for (int j = 0; j < t; j++)
+                    if ((p >> j) & 1u) y ^= inv[j];
+                return y;
+            };
+            std::fill(g, g + 16, 0ull);
+            for (int i = 0; i < nt; i++) g[i] = offset(tile_index(l.tcols[i]), pa.store_pos);
+            for (int k = 0; k < nb; k++) g[12 + k] = offset(tile_index(l.pcols[k]), pa.store_pos);
+            if (coalesced(g)) {
+                std::copy(g, g + 16, pa.last_g);
+                pa.fuse_last = 1;
+            }
+        }
+    }
+
     int micro_bit(int tile_bit, const uint8_t* bits) const
     {
         for (int j = 0; j < opt.micro_bits; j++)
@@ -674,7 +754,12 @@ struct Builder {
                 rem.erase(rem.begin());
                 continue;
             }
-            const uint64_t full = pad_set(set, nb, t);
+            // pad with tile bits above the low qubits first: the low qubits
+            // then stay on lane bits 0..low-1, which a direct first / last
+            // step needs for coalesced HBM runs
+            uint64_t full = set;
+            for (int b = std::min(opt.low_qubits, t); b < t && popc(full) < nb; b++) full |= 1ull << b;
+            full = pad_set(full, nb, t);
             ms.kind = SK_REG;
             ms.nbits = (uint8_t)nb;
             int nb = 0;
@@ -744,6 +829,7 @@ struct Builder {
         pa.has_wide = 0;
         for (uint32_t m = pa.micro_begin; m < plan.micro.size(); m++)
             if (plan.micro[m].kind != SK_REG) pa.has_wide = 1;
+        fill_direct(pa);
         return "";
     }
 
@@ -907,7 +993,11 @@ std::string Plan::describe(bool verbose) const
         }
         size_t nops = 0;
         for (uint32_t m = 0; m < p.micro_count; m++) nops += micro[p.micro_begin + m].op_count;
-        os << " micro=" << p.micro_count << " ops=" << nops << "\n";
+        os << " micro=" << p.micro_count << " ops=" << nops;
+        if (p.fuse_first || p.fuse_last)
+            os << " direct=" << (p.fuse_first ? "first" : "") << (p.fuse_first && p.fuse_last ? "," : "")
+               << (p.fuse_last ? "last" : "");
+        os << "\n";
         if (!verbose) continue;
         static const char* kinds[] = {"reg", "generic", "mfma", "mfma_bf16"};
         static const char* okinds[] = {"dense", "phase", "h", "cnot", "crk", "toff", "kron", "ekron"};

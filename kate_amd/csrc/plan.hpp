@@ -81,6 +81,17 @@ struct PassArgs {
     uint32_t micro_count;
     uint16_t store_cols[16];  // tile layout at the end of the pass (see MicroStep::cols)
     uint32_t store_byte[16];  // the same as swizzled LDS byte offsets
+    // Direct first / last micro-step: the first SK_REG step loads its
+    // amplitudes from HBM into registers (no LDS fill), the last one stores
+    // them to HBM from registers (no LDS drain).  first_g[i] / last_g[i] is
+    // the state offset contributed by thread-index bit i of that step
+    // (i < tile_bits - micro_bits), first_g[12 + k] / last_g[12 + k] that of
+    // register bit k; offsets combine by XOR with the tile base.
+    uint8_t fuse_first;
+    uint8_t fuse_last;
+    uint8_t pad_[2];
+    uint64_t first_g[16];
+    uint64_t last_g[16];
 };
 
 struct PlanGate {
@@ -116,6 +127,7 @@ struct PlanOptions {
     int micro_bits = 4;      // register micro-steps span 3 or 4 tile bits (8 or 16 amplitudes per thread)
     int amp_bytes = 16;      // 16 (complex128) or 8 (complex64): LDS byte offsets in the plan
     bool euler = true;       // Kronecker ops of 2+ composed 1-qubit gates in Euler form (OK_EKRON)
+    bool direct = true;      // first / last SK_REG step of a pass loads / stores HBM directly
 };
 
 struct Plan {

@@ -319,8 +319,9 @@ std::string plan_options(const kate_options& opt, kate::PlanOptions& po)
     po.mfma_bf16 = opt.use_mfma == 3;
     po.micro_bits = opt.micro_bits ? opt.micro_bits : 4;
     po.amp_bytes = opt.dtype == KATE_C128 ? 16 : 8;
-    if (opt.no_euler != 0 && opt.no_euler != 1) return "no_euler must be 0 or 1";
-    po.euler = opt.no_euler == 0;
+    if (opt.plan_flags & ~(KATE_PLAN_NO_EULER | KATE_PLAN_NO_DIRECT)) return "unknown plan_flags bits";
+    po.euler = !(opt.plan_flags & KATE_PLAN_NO_EULER);
+    po.direct = !(opt.plan_flags & KATE_PLAN_NO_DIRECT);
     if (po.micro_bits != 3 && po.micro_bits != 4) return "micro_bits must be 3 or 4";
     if (po.micro_bits == 3 && opt.dtype != KATE_C128) return "micro_bits 3 needs KATE_C128";
     if (po.micro_bits == 3 && opt.use_mfma) return "micro_bits 3 excludes use_mfma";

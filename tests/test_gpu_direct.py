@@ -1,0 +1,57 @@
+"""Direct first / last micro-steps (the first register step of a pass reads
+its amplitudes from HBM, the last one writes them back, without the LDS fill
+and drain): the same arithmetic as the plain pass, so states agree bit for
+bit with direct=False, and with the oracle to the usual tolerance.  Covers
+both register widths, every tile shape, complex64, lazy-CNOT layouts (the
+CNOT ladders), swap programs (remap passes) and batches."""
+import numpy as np
+import pytest
+
+import kate_amd as K
+from kate_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def state_of(c, dtype="c128", **opt):
+    with K.Simulator(c, dtype=dtype, **opt) as sim:
+        sim.execute()
+        return sim.state()
+
+
+CASES = [
+    ("layered", 14, lambda n: W.layered(n, 6, 21)),
+    ("mixed", 13, lambda n: W.random_mixed(n, 200, 22, max_dense=4) + W.layered(n, 2, 23)),
+    ("brick", 16, lambda n: W.brickwork(n, 8, 24, longrange_offset=8)),
+    ("haar", 15, lambda n: W.haar_layers(n, 4, 25)),
+    ("qft", 14, lambda n: W.qft(n)),
+]
+
+
+@pytest.mark.parametrize("dtype,mb", [("c128", 3), ("c128", 4), ("c64", 4)])
+@pytest.mark.parametrize("tile", [10, 12, 13])
+@pytest.mark.parametrize("name,n,gen", CASES, ids=[c[0] for c in CASES])
+def test_direct_steps_bit_exact(gpu, dtype, mb, tile, name, n, gen):
+    say("direct", dtype, mb, tile, name, n)
+    gates = gen(n)
+    _, _, text = K.Circuit(n, gates).plan(dtype, tile_qubits=tile, micro_bits=mb)
+    a = state_of(K.Circuit(n, gates), dtype, tile_qubits=tile, micro_bits=mb)
+    b = state_of(K.Circuit(n, gates), dtype, tile_qubits=tile, micro_bits=mb, direct=False)
+    assert np.array_equal(a, b), (name, "direct=" in text)
+    tol = 1e-12 if dtype == "c128" else 2e-6
+    assert np.max(np.abs(a - O.simulate(n, gates))) <= tol
+
+
+def test_direct_swap_program_and_20_qubits(gpu):
+    say("direct 20q swap program")
+    n = 20
+    gates = W.brickwork(n, 10, 27, longrange_offset=10)
+    for mk in (lambda: K.Circuit(n, gates), lambda: K.Circuit(n, gates).schedule().group(8).allocate(3)):
+        a = state_of(mk())
+        b = state_of(mk(), direct=False)
+        assert np.array_equal(a, b)

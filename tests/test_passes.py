@@ -183,3 +183,11 @@ def test_plan_euler_form_ops(monkeypatch):
     assert any(k == "ekron" and int(c) >= 16 for k, c in haar)
     # with the right diagonal, a 2-bit Euler op is no cheaper than the plain one
     assert all(k == "kron" or int(c) < 16 or bin(int(c) & 15).count("1") >= 3 for k, c in haar)
+
+
+def test_direct_plans_take_the_direct_path():
+    n = 20
+    _, _, text = K.Circuit(n, W.layered(n, 10, 26)).plan("c128")
+    lines = [ln for ln in text.splitlines() if ln.startswith("pass")]
+    assert sum("direct=first" in ln for ln in lines) >= len(lines) // 2
+    assert sum("last" in ln for ln in lines) >= len(lines) // 3
