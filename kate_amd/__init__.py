@@ -426,12 +426,12 @@ class Batch:
     starts at |0...0>; a run returns <Z_q> for every qubit of every circuit."""
 
     def __init__(self, circuits: Sequence[Circuit], dtype: str = "c64", tile_qubits: int = 0, low_qubits: int = 0,
-                 device: int = 0, mfma: int = 0, micro_bits: int = 0):
+                 device: int = 0, mfma: int = 0, micro_bits: int = 0, euler: bool = True, direct: bool = True):
         self.circuits = list(circuits)
         self.nq = [c.n for c in self.circuits]
         arr = (VP * len(self.circuits))(*[c._h for c in self.circuits])
         opt = Options({"c128": KATE_C128, "c64": KATE_C64}[dtype], tile_qubits, low_qubits, mfma, 0, device,
-                      micro_bits)
+                      micro_bits, _plan_flags(euler, direct))
         h = VP()
         _check(lib().kate_batch_create_from_circuits(arr, len(self.circuits), C.byref(opt), C.byref(h)),
                "kate_batch_create")

@@ -569,6 +569,15 @@ __device__ __forceinline__ void reg_step_ops(Cx<R> (&v)[N], const MicroStep* __r
     }
 }
 
+// Amplitude at state offset `off` as a pass reads it: from HBM, or, on the
+// first pass after a basis-state init, made in registers.
+template <typename R>
+__device__ __forceinline__ Cx<R> pass_load(const Cx<R>* __restrict__ state, const PassArgs& pa, uint64_t off)
+{
+    if (pa.synth_basis) return Cx<R>{off == pa.synth_index ? (R)1 : (R)0, (R)0};
+    return state[off];
+}
+
 // State offset of register amplitude s of this thread for a direct step:
 // the XOR of the columns of the set thread-index and register bits.
 template <int TB, int NB>
@@ -689,10 +698,15 @@ __device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassA
         const uint64_t gt = base ^ direct_thread_offset<TB, NB>(pa.first_g, tid);
         Cx<R> v[N];
 #pragma unroll
-        for (int s = 0; s < N; s++) v[s] = state[gt ^ direct_reg_offset<NB>(pa.first_g, s)];
+        for (int s = 0; s < N; s++) v[s] = pass_load(state, pa, gt ^ direct_reg_offset<NB>(pa.first_g, s));
         reg_step_ops<R, N>(v, msp, op_begin, op_count, ops, coef);
         m_begin = 1;
         if (pa.micro_count == 1 && pa.fuse_last) {
+            // the store of one thread can hit an amplitude that another
+            // thread of the tile loads (the final layout moves amplitudes
+            // between threads): every load of the tile has completed (its
+            // value was used above) once all threads pass this barrier
+            __syncthreads();
             const uint64_t gs = base ^ direct_thread_offset<TB, NB>(pa.last_g, tid);
 #pragma unroll
             for (int s = 0; s < N; s++) state[gs ^ direct_reg_offset<NB>(pa.last_g, s)] = v[s];
@@ -714,7 +728,7 @@ __device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassA
 #pragma unroll
             for (int k = 0; k < NB; k++)
                 if ((i >> k) & 1) off += m[k];
-            r[i] = state[off];
+            r[i] = pass_load(state, pa, off);
         }
         const uint32_t tsw = swz<R>(tid) * (uint32_t)sizeof(Cx<R>);
 #pragma unroll
@@ -760,8 +774,14 @@ __global__ __launch_bounds__(1 << (TB - NB)) void k_pass(Cx<R>* __restrict__ sta
 // queue order and only wait on earlier ones, which running workgroups hold,
 // so the queue always drains; the wait is still bounded, and a wait that
 // runs out sets *err instead of spinning on.
+// complex64 register-step batch kernels are held to 128 VGPRs (4 waves per
+// SIMD): with 137 the 8-wave workgroups of 13-bit tiles ran one per CU.
+template <typename R, bool WIDE>
+constexpr int kBatchWavesPerEU = (sizeof(R) == 4 && !WIDE) ? 4 : 1;
+
 template <typename R, int TB, bool WIDE, int NB>
-__global__ __launch_bounds__(1 << (TB - NB)) void k_batch(Cx<R>* __restrict__ states,
+__global__ __launch_bounds__(1 << (TB - NB)) __attribute__((amdgpu_waves_per_eu(kBatchWavesPerEU<R, WIDE>)))
+void k_batch(Cx<R>* __restrict__ states,
                                                          const uint64_t* __restrict__ circ_off,
                                                          const BatchTask* __restrict__ tasks, uint32_t ntasks,
                                                          uint32_t* __restrict__ next, uint32_t* __restrict__ done,

@@ -1019,6 +1019,17 @@ std::string Plan::describe(bool verbose) const
             }
             os << "\n";
         }
+        os << "    store_cols=";
+        for (int b = 0; b < p.tile_bits; b++) os << (b ? "," : "") << p.store_cols[b];
+        if (p.fuse_first) {
+            os << " first_g=";
+            for (int i = 0; i < 16; i++) os << (i ? "," : "") << p.first_g[i];
+        }
+        if (p.fuse_last) {
+            os << " last_g=";
+            for (int i = 0; i < 16; i++) os << (i ? "," : "") << p.last_g[i];
+        }
+        os << "\n";
     }
     return os.str();
 }

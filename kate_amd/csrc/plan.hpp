@@ -89,9 +89,14 @@ struct PassArgs {
     // register bit k; offsets combine by XOR with the tile base.
     uint8_t fuse_first;
     uint8_t fuse_last;
-    uint8_t pad_[2];
+    // Set per launch on the first pass of an execute from a basis state: the
+    // tile is not read from HBM but made in registers (amplitude 1 at state
+    // offset synth_index, 0 elsewhere), which replaces the init pass.
+    uint8_t synth_basis;
+    uint8_t pad_;
     uint64_t first_g[16];
     uint64_t last_g[16];
+    uint64_t synth_index;
 };
 
 struct PlanGate {

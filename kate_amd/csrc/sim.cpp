@@ -204,10 +204,18 @@ void release(kate_sim* s)
     delete s;
 }
 
-int run_passes(kate_sim* s)
+// Runs every pass.  With `basis` set, the state is taken to be the basis
+// state |basis_index> without being read: the first pass makes its tiles in
+// registers (PassArgs::synth_basis), so the caller skips the init kernel.
+int run_passes(kate_sim* s, bool basis = false, uint64_t basis_index = 0)
 {
     kate::DevPlan dp{s->d_micro, s->d_ops, s->d_coef, s->d_raw};
-    for (const kate::PassArgs& pa : s->plan.passes) {
+    for (size_t i = 0; i < s->plan.passes.size(); i++) {
+        kate::PassArgs pa = s->plan.passes[i];
+        if (basis && i == 0) {
+            pa.synth_basis = 1;
+            pa.synth_index = basis_index;
+        }
         int e = kate::launch_pass(s->dtype, s->plan.tile_bits, s->d_state, s->nq_int, pa, dp, s->stream);
         if (e) return fail(KATE_ERR_HIP, std::string("pass launch: ") + hipGetErrorString((hipError_t)e));
     }
@@ -565,10 +573,15 @@ int kate_sim_execute(kate_sim* s, const double* tp_in, double* tp_out)
     if (!s) return fail(KATE_ERR_ARG, "null simulator");
     KATE_HIP(hipSetDevice(s->device));
     KATE_HIP(hipEventRecord(s->ev[0], s->stream));
-    int rc = tp_in ? init_product(s, tp_in) : kate::launch_init_basis(s->dtype, s->d_state, s->nq_int, 0, s->stream);
+    // |0...0> is made by the first pass itself (no init pass over HBM) unless
+    // there is no pass to make it
+    const bool fused_init = !tp_in && !s->plan.passes.empty();
+    int rc = KATE_OK;
+    if (tp_in) rc = init_product(s, tp_in);
+    else if (!fused_init) rc = kate::launch_init_basis(s->dtype, s->d_state, s->nq_int, 0, s->stream);
     if (rc) return rc > 0 ? fail(KATE_ERR_HIP, "init launch failed") : rc;
     KATE_HIP(hipEventRecord(s->ev[1], s->stream));
-    rc = run_passes(s);
+    rc = run_passes(s, fused_init, 0);
     if (rc) return rc;
     KATE_HIP(hipEventRecord(s->ev[2], s->stream));
     std::vector<double> tmp(2 * (size_t)s->nq);

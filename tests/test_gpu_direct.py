@@ -55,3 +55,40 @@ def test_direct_swap_program_and_20_qubits(gpu):
         a = state_of(mk())
         b = state_of(mk(), direct=False)
         assert np.array_equal(a, b)
+
+
+def test_direct_single_step_pass_at_28_qubits(gpu):
+    """The 28-qubit layered plan has a pass of one micro-step, loaded and
+    stored directly: the store of one wave must not overtake a load of
+    another wave of the tile (the final layout moves amplitudes between
+    threads).  Compared on the device with the plain plan, which shares no
+    code path with it in that pass."""
+    say("direct 28q single-step pass")
+    n = 28
+    gates = W.layered(n, 20, 2)
+    _, _, text = K.Circuit(n, gates).plan("c128")
+    assert any(" micro=1 " in ln and "direct=first,last" in ln for ln in text.splitlines())
+    with K.Simulator(K.Circuit(n, gates), dtype="c128") as a, \
+            K.Simulator(K.Circuit(n, gates), dtype="c128", direct=False) as b:
+        a.execute()
+        b.execute()
+        _, na, nb, mx = a.compare(b)
+    assert mx <= 1e-14 and abs(na - 1.0) <= 1e-12 and abs(nb - 1.0) <= 1e-12
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_execute_makes_the_basis_state_in_its_first_pass(gpu, dtype):
+    """execute() from |0...0> lets the first pass make its tiles in registers
+    instead of running the init kernel and reading the state back: the same
+    state, bit for bit, as init_basis(0) followed by run()."""
+    say("fused init", dtype)
+    for n, gates in ((14, W.random_mixed(14, 120, 31, max_dense=4)), (20, W.layered(20, 6, 32)),
+                     (18, W.brickwork(18, 6, 33, longrange_offset=9))):
+        for mk in (lambda: K.Circuit(n, gates), lambda: K.Circuit(n, gates).schedule().group(8).allocate(2)):
+            with K.Simulator(mk(), dtype=dtype) as sim:
+                sim.execute()
+                a = sim.state()
+                sim.init_basis(0)
+                sim.run()
+                b = sim.state()
+            assert np.array_equal(a, b)
