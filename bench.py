@@ -29,9 +29,12 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_pass launch on the default workload (30 qubits, depth 20),
 # from rocprofv3 FETCH_SIZE and WRITE_SIZE runs, each scaled by what the same
 # counter reads on the gate-free passes of the same run (plain tile copies of
-# 16 GiB read + 16 GiB written: factors 2.000 and 1.000): 1.00001 x the
-# algorithmic bytes (profiles/k_pass_f64_30q_layered_hbm_counters.md).
-TRAFFIC_RATIO_30Q_LAYERED = 1.00001
+# 16 GiB read + 16 GiB written: factors 2.000 and 1.000), averaged over the
+# 27 passes of one execute and the 27 of one pass-by-pass run: 0.99089 x the
+# algorithmic bytes -- every pass reads and writes the state once, except
+# that pass 0 of an execute makes |0...0> in registers and reads nothing
+# (profiles/k_pass_f64_30q_layered_hbm_counters.md).
+TRAFFIC_RATIO_30Q_LAYERED = 0.99089
 METRIC = "30q circuit ms + HBM % peak; served circuits/s + p99 at fidelity≥1-1e-6"
 
 

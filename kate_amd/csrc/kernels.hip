@@ -569,13 +569,12 @@ __device__ __forceinline__ void reg_step_ops(Cx<R> (&v)[N], const MicroStep* __r
     }
 }
 
-// Amplitude at state offset `off` as a pass reads it: from HBM, or, on the
-// first pass after a basis-state init, made in registers.
+// Amplitude at state offset `off` of the basis state that the first pass
+// of an execute makes in registers instead of reading (PassArgs::synth_basis).
 template <typename R>
-__device__ __forceinline__ Cx<R> pass_load(const Cx<R>* __restrict__ state, const PassArgs& pa, uint64_t off)
+__device__ __forceinline__ Cx<R> synth_amp(const PassArgs& pa, uint64_t off)
 {
-    if (pa.synth_basis) return Cx<R>{off == pa.synth_index ? (R)1 : (R)0, (R)0};
-    return state[off];
+    return Cx<R>{off == pa.synth_index ? (R)1 : (R)0, (R)0};
 }
 
 // State offset of register amplitude s of this thread for a direct step:
@@ -697,8 +696,13 @@ __device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassA
         step_addressing<TB, NB>(msp, tid, op_begin, op_count, oc, b);
         const uint64_t gt = base ^ direct_thread_offset<TB, NB>(pa.first_g, tid);
         Cx<R> v[N];
+        if (pa.synth_basis) {
 #pragma unroll
-        for (int s = 0; s < N; s++) v[s] = pass_load(state, pa, gt ^ direct_reg_offset<NB>(pa.first_g, s));
+            for (int s = 0; s < N; s++) v[s] = synth_amp<R>(pa, gt ^ direct_reg_offset<NB>(pa.first_g, s));
+        } else {
+#pragma unroll
+            for (int s = 0; s < N; s++) v[s] = state[gt ^ direct_reg_offset<NB>(pa.first_g, s)];
+        }
         reg_step_ops<R, N>(v, msp, op_begin, op_count, ops, coef);
         m_begin = 1;
         if (pa.micro_count == 1 && pa.fuse_last) {
@@ -722,13 +726,24 @@ __device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassA
 #pragma unroll
         for (int k = 0; k < NB; k++) m[k] = 1ull << pa.load_pos[TB - NB + k];
         Cx<R> r[N];
+        if (pa.synth_basis) {
 #pragma unroll
-        for (int i = 0; i < N; i++) {
-            uint64_t off = tb;
+            for (int i = 0; i < N; i++) {
+                uint64_t off = tb;
 #pragma unroll
-            for (int k = 0; k < NB; k++)
-                if ((i >> k) & 1) off += m[k];
-            r[i] = pass_load(state, pa, off);
+                for (int k = 0; k < NB; k++)
+                    if ((i >> k) & 1) off += m[k];
+                r[i] = synth_amp<R>(pa, off);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+                uint64_t off = tb;
+#pragma unroll
+                for (int k = 0; k < NB; k++)
+                    if ((i >> k) & 1) off += m[k];
+                r[i] = state[off];
+            }
         }
         const uint32_t tsw = swz<R>(tid) * (uint32_t)sizeof(Cx<R>);
 #pragma unroll
