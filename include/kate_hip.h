@@ -84,7 +84,9 @@ typedef struct kate_program {
 typedef struct kate_options {
     int32_t dtype;        /* kate_dtype; KATE_C128 is the plain path */
     int32_t tile_qubits;  /* 10..13; 0 = 12 */
-    int32_t low_qubits;   /* qubits always in a tile (coalesced run = 2^low amplitudes); 0 = default */
+    int32_t low_qubits;   /* qubits always in a tile (coalesced run = 2^low amplitudes); 0 = the
+                             planner tries 2-5 (complex128) / 3-6 (complex64) and keeps the plan
+                             with the lowest estimated HBM + register-step cost */
     int32_t use_mfma;     /* C64 only: 1 = dense gates of 4-5 qubits on the matrix cores;
                              2 = also compose the gates of each 5-qubit window into one
                              dense gate (FP64 composition) for the matrix cores;

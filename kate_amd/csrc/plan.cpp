@@ -940,9 +940,55 @@ struct Builder {
 
 }  // namespace
 
+// Relative HBM time of a pass whose tiles are read and written in runs of
+// `run_bytes` contiguous bytes (one MI355X, 30-qubit passes: 64 B runs cost
+// 1.37x, 256 B runs 0.92x the time of 128 B runs).
+static double run_cost(int run_bytes)
+{
+    if (run_bytes <= 32) return 2.5;
+    if (run_bytes <= 64) return 1.37;
+    if (run_bytes <= 128) return 1.0;
+    if (run_bytes <= 256) return 0.92;
+    return 0.9;
+}
+
 std::string build_plan(const Program& prog, const PlanOptions& opt, Plan& plan)
 {
+    if (opt.auto_low) {
+        // More low qubits per tile give longer HBM runs but fewer free tile
+        // qubits, hence more passes; the light-cone packer's pass count is
+        // not monotonic in it (28-qubit layered circuit: 32 passes with 3
+        // low qubits, 28 with 4).  Plan each candidate and keep the cheapest
+        // by pass count x run cost + 0.1 pass per micro-step (measured: a
+        // register step costs about a tenth of a 128 B-run pass).
+        PlanOptions o = opt;
+        o.auto_low = false;
+        std::string first_err;
+        double best = 0;
+        bool have = false;
+        const int lo = opt.amp_bytes == 16 ? 2 : 3;
+        for (int low = lo; low <= lo + 3; low++) {
+            if (low > std::min(opt.tile_bits, kMaxQubits) - opt.micro_bits) break;
+            o.low_qubits = low;
+            Plan cand;
+            std::string e = build_plan(prog, o, cand);
+            if (!e.empty()) {
+                if (first_err.empty()) first_err = e;
+                continue;
+            }
+            const int run = (1 << std::min(low, cand.nq_int)) * opt.amp_bytes;
+            const double cost = (double)cand.passes.size() * run_cost(run) + 0.1 * (double)cand.micro.size();
+            if (!have || cost < best) {
+                best = cost;
+                plan = std::move(cand);
+                have = true;
+            }
+        }
+        if (!have) return first_err.empty() ? "no plan" : first_err;
+        return "";
+    }
     plan = Plan();
+    plan.low_qubits = opt.low_qubits;
     plan.nq_int = std::max(prog.nqubits, opt.min_qubits);
     if (plan.nq_int > kMaxQubits) return "too many qubits";
     plan.tile_bits = std::min(opt.tile_bits, plan.nq_int);
@@ -979,7 +1025,8 @@ std::string build_plan(const Program& prog, const PlanOptions& opt, Plan& plan)
 std::string Plan::describe(bool verbose) const
 {
     std::ostringstream os;
-    os << "nq_int=" << nq_int << " tile_bits=" << tile_bits << " passes=" << passes.size() << " (gate " << gate_passes
+    os << "nq_int=" << nq_int << " tile_bits=" << tile_bits << " low=" << low_qubits << " passes=" << passes.size()
+       << " (gate " << gate_passes
        << ", remap " << remap_passes << ") micro=" << micro.size() << " ops=" << ops.size() << "\n";
     for (size_t i = 0; i < passes.size(); i++) {
         const PassArgs& p = passes[i];

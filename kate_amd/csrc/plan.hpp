@@ -133,6 +133,7 @@ struct PlanOptions {
     int amp_bytes = 16;      // 16 (complex128) or 8 (complex64): LDS byte offsets in the plan
     bool euler = true;       // Kronecker ops of 2+ composed 1-qubit gates in Euler form (OK_EKRON)
     bool direct = true;      // first / last SK_REG step of a pass loads / stores HBM directly
+    bool auto_low = false;   // build the plan for several low_qubits and keep the cheapest
 };
 
 struct Plan {
@@ -146,6 +147,7 @@ struct Plan {
     std::vector<uint32_t> raw; // matrix-core operands, uploaded bit for bit
     int gate_passes = 0;
     int remap_passes = 0;
+    int low_qubits = 0;  // qubits 0..low-1 in every gate tile (as planned)
     std::string describe(bool verbose = false) const;
 };
 

@@ -318,6 +318,7 @@ std::string plan_options(const kate_options& opt, kate::PlanOptions& po)
     if (opt.dtype != KATE_C128 && opt.dtype != KATE_C64) return "dtype must be KATE_C128 or KATE_C64";
     po.tile_bits = opt.tile_qubits ? opt.tile_qubits : 12;
     po.low_qubits = opt.low_qubits ? opt.low_qubits : (opt.dtype == KATE_C128 ? 3 : 4);
+    po.auto_low = opt.low_qubits == 0;  // 0: the planner picks it per circuit
     // the reductions read whole 4096-amplitude blocks
     po.min_qubits = std::max(po.tile_bits, 12);
     if (opt.use_mfma && opt.dtype != KATE_C64) return "use_mfma needs KATE_C64 (the FP32 matrix-core path)";

@@ -58,7 +58,7 @@ def test_direct_swap_program_and_20_qubits(gpu):
 
 
 def test_direct_single_step_pass_at_28_qubits(gpu):
-    """The 28-qubit layered plan has a pass of one micro-step, loaded and
+    """The 28-qubit layered plan with 3 low qubits has a pass of one micro-step, loaded and
     stored directly: the store of one wave must not overtake a load of
     another wave of the tile (the final layout moves amplitudes between
     threads).  Compared on the device with the plain plan, which shares no
@@ -66,10 +66,10 @@ def test_direct_single_step_pass_at_28_qubits(gpu):
     say("direct 28q single-step pass")
     n = 28
     gates = W.layered(n, 20, 2)
-    _, _, text = K.Circuit(n, gates).plan("c128")
+    _, _, text = K.Circuit(n, gates).plan("c128", low_qubits=3)
     assert any(" micro=1 " in ln and "direct=first,last" in ln for ln in text.splitlines())
-    with K.Simulator(K.Circuit(n, gates), dtype="c128") as a, \
-            K.Simulator(K.Circuit(n, gates), dtype="c128", direct=False) as b:
+    with K.Simulator(K.Circuit(n, gates), dtype="c128", low_qubits=3) as a, \
+            K.Simulator(K.Circuit(n, gates), dtype="c128", low_qubits=3, direct=False) as b:
         a.execute()
         b.execute()
         _, na, nb, mx = a.compare(b)
