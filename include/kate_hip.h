@@ -108,7 +108,11 @@ enum {
     /* fill and drain every LDS tile through LDS instead of loading the
        first micro-step's amplitudes from HBM into registers and storing the
        last one's from registers (same arithmetic, bit for bit) */
-    KATE_PLAN_NO_DIRECT = 2
+    KATE_PLAN_NO_DIRECT = 2,
+    /* execute() from |0...0>: read and launch every tile instead of only
+       those that can hold non-zero amplitudes (qubits no gate has touched
+       yet keep every amplitude with their bit set at exactly zero) */
+    KATE_PLAN_NO_SUPPORT = 4
 };
 
 /* Wall-clock split of the last execute, as Kate's execute(true) prints it

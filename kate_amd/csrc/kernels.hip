@@ -577,6 +577,14 @@ __device__ __forceinline__ Cx<R> synth_amp(const PassArgs& pa, uint64_t off)
     return Cx<R>{off == pa.synth_index ? (R)1 : (R)0, (R)0};
 }
 
+// Amplitude at `off` in a run from |0...0>: zero without a read when the
+// offset has a bit of a qubit that no gate has touched yet.
+template <typename R>
+__device__ __forceinline__ Cx<R> masked_load(const Cx<R>* __restrict__ state, const PassArgs& pa, uint64_t off)
+{
+    return (off & pa.zero_mask) ? Cx<R>{(R)0, (R)0} : state[off];
+}
+
 // State offset of register amplitude s of this thread for a direct step:
 // the XOR of the columns of the set thread-index and register bits.
 template <int TB, int NB>
@@ -699,6 +707,9 @@ __device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassA
         if (pa.synth_basis) {
 #pragma unroll
             for (int s = 0; s < N; s++) v[s] = synth_amp<R>(pa, gt ^ direct_reg_offset<NB>(pa.first_g, s));
+        } else if (pa.zero_mask) {
+#pragma unroll
+            for (int s = 0; s < N; s++) v[s] = masked_load(state, pa, gt ^ direct_reg_offset<NB>(pa.first_g, s));
         } else {
 #pragma unroll
             for (int s = 0; s < N; s++) v[s] = state[gt ^ direct_reg_offset<NB>(pa.first_g, s)];
@@ -734,6 +745,15 @@ __device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassA
                 for (int k = 0; k < NB; k++)
                     if ((i >> k) & 1) off += m[k];
                 r[i] = synth_amp<R>(pa, off);
+            }
+        } else if (pa.zero_mask) {
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+                uint64_t off = tb;
+#pragma unroll
+                for (int k = 0; k < NB; k++)
+                    if ((i >> k) & 1) off += m[k];
+                r[i] = masked_load(state, pa, off);
             }
         } else {
 #pragma unroll
@@ -1146,7 +1166,9 @@ inline unsigned grid_for(uint64_t n)
 template <typename R, int TB, int NB>
 int pass_t(void* state, int nq_int, const PassArgs& pa, const DevPlan& dp, hipStream_t s)
 {
-    const unsigned grid = 1u << (nq_int - TB);
+    // one workgroup per tile; a run from |0...0> may list fewer rest qubits
+    (void)nq_int;
+    const unsigned grid = 1u << pa.nrest;
     if (pa.has_wide)
         hipLaunchKernelGGL((k_pass<R, TB, true, NB>), dim3(grid), dim3(1 << (TB - NB)), 0, s, (Cx<R>*)state, pa,
                            dp.micro, dp.ops, (const Cx<R>*)dp.coef, dp.raw);
@@ -1271,7 +1293,8 @@ int launch_overlap(int dtype_a, const void* a, int dtype_b, const void* b, int n
 
 int launch_pass(int dtype, int tile_bits, void* state, int nq_int, const PassArgs& pa, const DevPlan& dp, void* stream)
 {
-    if (pa.tile_bits != tile_bits || nq_int < tile_bits || nq_int - tile_bits > 31) return (int)hipErrorInvalidValue;
+    if (pa.tile_bits != tile_bits || nq_int < tile_bits || nq_int - tile_bits > 31 || pa.nrest > nq_int - tile_bits)
+        return (int)hipErrorInvalidValue;
     if (dtype == 0) return pass_r<double>(tile_bits, state, nq_int, pa, dp, (hipStream_t)stream);
     return pass_r<float>(tile_bits, state, nq_int, pa, dp, (hipStream_t)stream);
 }

@@ -169,6 +169,40 @@ struct Builder {
     const Program& prog;
     const PlanOptions& opt;
     Plan& plan;
+    uint64_t zero = ~0ull;  // qubits still exactly |0> in a run from |0...0>
+    bool remapped = false;  // a remap pass was planned: support no longer tracked
+
+    // Support update for one gate (sound: a qubit stays in `zero` only if
+    // every amplitude with that bit set is still exactly zero).
+    void track(const PlanGate& g)
+    {
+        switch (g.id) {
+        case 3:  // CNOT {target, control}: moves weight only if the control can be 1
+            if (!((zero >> g.q[1]) & 1ull)) zero &= ~(1ull << g.q[0]);
+            break;
+        case 5:  // TOFFOLI {c0, c1, target}
+            if (!((zero >> g.q[0]) & 1ull) && !((zero >> g.q[1]) & 1ull)) zero &= ~(1ull << g.q[2]);
+            break;
+        case 4:  // CRk: diagonal
+            break;
+        case 0: {
+            const uint64_t dim = 1ull << g.k;
+            const double* m = prog.mat.data() + 2 * g.moff;
+            bool diag = true;
+            for (uint64_t r = 0; r < dim && diag; r++)
+                for (uint64_t c = 0; c < dim; c++)
+                    if (r != c && (m[2 * (r * dim + c)] != 0.0 || m[2 * (r * dim + c) + 1] != 0.0)) {
+                        diag = false;
+                        break;
+                    }
+            if (!diag) zero &= ~g.mask;
+            break;
+        }
+        default:  // H and anything else: its qubits can be 1 afterwards
+            zero &= ~g.mask;
+            break;
+        }
+    }
 
     uint32_t add_coef(const double* c, size_t ncomplex)
     {
@@ -881,6 +915,9 @@ struct Builder {
             for (int a : acc) gl.push_back(rem[(size_t)a]);
             std::string e = pack_micro(gl, tile_of, pa);
             if (!e.empty()) return e;
+            plan.pass_zero.push_back(remapped ? 0ull : zero);
+            if (!remapped) plan.support_passes = (int)plan.passes.size() + 1;
+            for (int gi : gl) track(prog.gates[(size_t)gi]);
             plan.passes.push_back(pa);
             plan.pass_is_remap.push_back(0);
             plan.gate_passes++;
@@ -929,6 +966,11 @@ struct Builder {
             pa.micro_count = 0;
             for (int b = 0; b < 16; b++) pa.store_cols[b] = (uint16_t)(1u << b);
             fill_store_bytes(pa);
+            // a remap pass writes every tile (masked reads of the zero
+            // qubits); the support is not tracked through it
+            plan.pass_zero.push_back(remapped ? 0ull : zero);
+            remapped = true;
+            zero = 0;
             plan.passes.push_back(pa);
             plan.pass_is_remap.push_back(1);
             plan.remap_passes++;
@@ -996,6 +1038,7 @@ std::string build_plan(const Program& prog, const PlanOptions& opt, Plan& plan)
     if (opt.amp_bytes != 16 && opt.amp_bytes != 8) return "amplitudes have 8 or 16 bytes";
     if (opt.micro_bits != 4 && (opt.mfma || opt.mfma_fuse)) return "matrix-core steps need 4-bit micro-steps";
     Builder b{prog, opt, plan};
+    b.zero = plan.nq_int >= 64 ? ~0ull : ((1ull << plan.nq_int) - 1ull);
     std::vector<int> seg;
     size_t grp = 0;
     int first = 0;

@@ -97,6 +97,9 @@ struct PassArgs {
     uint64_t first_g[16];
     uint64_t last_g[16];
     uint64_t synth_index;
+    // Set per launch in a run from |0...0>: amplitudes whose state offset
+    // meets zero_mask are known to be zero and are not read.
+    uint64_t zero_mask;
 };
 
 struct PlanGate {
@@ -134,6 +137,7 @@ struct PlanOptions {
     bool euler = true;       // Kronecker ops of 2+ composed 1-qubit gates in Euler form (OK_EKRON)
     bool direct = true;      // first / last SK_REG step of a pass loads / stores HBM directly
     bool auto_low = false;   // build the plan for several low_qubits and keep the cheapest
+    bool support = true;     // runs from |0...0> skip tiles known to be zero (used by the simulator)
 };
 
 struct Plan {
@@ -148,6 +152,13 @@ struct Plan {
     int gate_passes = 0;
     int remap_passes = 0;
     int low_qubits = 0;  // qubits 0..low-1 in every gate tile (as planned)
+    // Support of a run that starts from |0...0>: pass_zero[i] holds the
+    // qubits q for which every amplitude with bit q set is exactly zero when
+    // pass i starts (no gate has yet moved weight onto q = 1).  Passes
+    // before the first remap pass (support_passes of them) may skip the
+    // tiles whose non-tile qubits meet that set.
+    std::vector<uint64_t> pass_zero;
+    int support_passes = 0;
     std::string describe(bool verbose = false) const;
 };
 

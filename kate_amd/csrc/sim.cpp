@@ -169,6 +169,7 @@ struct kate_sim {
     std::vector<double> h_factors;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     kate_timings last{};
+    bool support = true;  // execute() from |0...0> skips tiles known to be zero
 };
 
 namespace {
@@ -207,14 +208,33 @@ void release(kate_sim* s)
 // Runs every pass.  With `basis` set, the state is taken to be the basis
 // state |basis_index> without being read: the first pass makes its tiles in
 // registers (PassArgs::synth_basis), so the caller skips the init kernel.
+// From |0...0> the passes also follow the plan's support (Plan::pass_zero):
+// a pass reads no amplitude that is known to be zero, and a pass before the
+// first remap that is not the last one launches only the tiles whose rest
+// qubits are all outside the zero set -- every other tile holds zeros in
+// and out, and no later pass reads it before a pass has written it (a rest
+// qubit keeps its zero status through the pass; the last pass writes every
+// tile, so the final state is whole).
 int run_passes(kate_sim* s, bool basis = false, uint64_t basis_index = 0)
 {
     kate::DevPlan dp{s->d_micro, s->d_ops, s->d_coef, s->d_raw};
-    for (size_t i = 0; i < s->plan.passes.size(); i++) {
+    const bool support = basis && basis_index == 0 && s->support;
+    const size_t np = s->plan.passes.size();
+    for (size_t i = 0; i < np; i++) {
         kate::PassArgs pa = s->plan.passes[i];
         if (basis && i == 0) {
             pa.synth_basis = 1;
             pa.synth_index = basis_index;
+        }
+        if (support && i < s->plan.pass_zero.size()) {
+            const uint64_t z = s->plan.pass_zero[i];
+            if (i > 0) pa.zero_mask = z;
+            if ((int)i < s->plan.support_passes && i + 1 < np) {
+                int k = 0;
+                for (int j = 0; j < pa.nrest; j++)
+                    if (!((z >> pa.rest_pos[j]) & 1ull)) pa.rest_pos[k++] = pa.rest_pos[j];
+                pa.nrest = (uint8_t)k;
+            }
         }
         int e = kate::launch_pass(s->dtype, s->plan.tile_bits, s->d_state, s->nq_int, pa, dp, s->stream);
         if (e) return fail(KATE_ERR_HIP, std::string("pass launch: ") + hipGetErrorString((hipError_t)e));
@@ -328,9 +348,11 @@ std::string plan_options(const kate_options& opt, kate::PlanOptions& po)
     po.mfma_bf16 = opt.use_mfma == 3;
     po.micro_bits = opt.micro_bits ? opt.micro_bits : 4;
     po.amp_bytes = opt.dtype == KATE_C128 ? 16 : 8;
-    if (opt.plan_flags & ~(KATE_PLAN_NO_EULER | KATE_PLAN_NO_DIRECT)) return "unknown plan_flags bits";
+    if (opt.plan_flags & ~(KATE_PLAN_NO_EULER | KATE_PLAN_NO_DIRECT | KATE_PLAN_NO_SUPPORT))
+        return "unknown plan_flags bits";
     po.euler = !(opt.plan_flags & KATE_PLAN_NO_EULER);
     po.direct = !(opt.plan_flags & KATE_PLAN_NO_DIRECT);
+    po.support = !(opt.plan_flags & KATE_PLAN_NO_SUPPORT);
     if (po.micro_bits != 3 && po.micro_bits != 4) return "micro_bits must be 3 or 4";
     if (po.micro_bits == 3 && opt.dtype != KATE_C128) return "micro_bits 3 needs KATE_C128";
     if (po.micro_bits == 3 && opt.use_mfma) return "micro_bits 3 excludes use_mfma";
@@ -449,6 +471,7 @@ int kate_sim_create(const kate_program* program, const kate_options* options, ka
         delete s;
         return fail(KATE_ERR_ARG, err);
     }
+    s->support = po.support;
     err = kate::build_plan(s->prog, po, s->plan);
     if (!err.empty()) {
         delete s;

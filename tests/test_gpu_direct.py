@@ -79,11 +79,18 @@ def test_direct_single_step_pass_at_28_qubits(gpu):
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
 def test_execute_makes_the_basis_state_in_its_first_pass(gpu, dtype):
     """execute() from |0...0> lets the first pass make its tiles in registers
-    instead of running the init kernel and reading the state back: the same
-    state, bit for bit, as init_basis(0) followed by run()."""
+    instead of running the init kernel, and follows the support of the state
+    (tiles of qubits no gate has touched yet are neither read nor launched
+    until a pass needs them; the last pass writes every tile): the same
+    state, bit for bit, as init_basis(0) followed by run(), including
+    circuits that never touch some qubits and CNOTs whose controls are still
+    |0>."""
     say("fused init", dtype)
+    partial = W.random_mixed(10, 150, 34, max_dense=3)  # qubits 10..17 never touched
+    ctrl = [W.Gate(W.CNOT, (q, q + 8)) for q in range(8)] + W.layered(16, 2, 35)  # controls still |0>
     for n, gates in ((14, W.random_mixed(14, 120, 31, max_dense=4)), (20, W.layered(20, 6, 32)),
-                     (18, W.brickwork(18, 6, 33, longrange_offset=9))):
+                     (18, W.brickwork(18, 6, 33, longrange_offset=9)), (18, partial), (16, ctrl),
+                     (16, W.qft(16)), (17, W.haar_layers(17, 3, 36))):
         for mk in (lambda: K.Circuit(n, gates), lambda: K.Circuit(n, gates).schedule().group(8).allocate(2)):
             with K.Simulator(mk(), dtype=dtype) as sim:
                 sim.execute()
@@ -92,3 +99,17 @@ def test_execute_makes_the_basis_state_in_its_first_pass(gpu, dtype):
                 sim.run()
                 b = sim.state()
             assert np.array_equal(a, b)
+
+
+def test_support_skipping_at_30_qubits(gpu):
+    """The 30-qubit layered circuit from |0...0>: execute() with the support
+    skipping against execute() without it, on the device."""
+    say("support 30q")
+    n = 30
+    gates = W.layered(n, 20, 1)
+    with K.Simulator(K.Circuit(n, gates), dtype="c128") as a, \
+            K.Simulator(K.Circuit(n, gates), dtype="c128", support=False) as b:
+        a.execute()
+        b.execute()
+        _, na, nb, mx = a.compare(b)
+    assert mx == 0.0 and abs(na - 1.0) <= 1e-12
