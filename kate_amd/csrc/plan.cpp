@@ -982,6 +982,29 @@ struct Builder {
 
 }  // namespace
 
+PassArgs zero_start_pass(const Plan& plan, size_t i, bool support)
+{
+    PassArgs pa = plan.passes[i];
+    if (i == 0) {
+        pa.synth_basis = 1;
+        pa.synth_index = 0;
+    }
+    if (support && i < plan.pass_zero.size()) {
+        // Every tile that is skipped is zero in and out, and no later pass
+        // reads it before a pass has written it: a rest qubit keeps its zero
+        // status through a pass, and the last pass writes every tile.
+        const uint64_t z = plan.pass_zero[i];
+        if (i > 0) pa.zero_mask = z;
+        if ((int)i < plan.support_passes && i + 1 < plan.passes.size()) {
+            int k = 0;
+            for (int j = 0; j < pa.nrest; j++)
+                if (!((z >> pa.rest_pos[j]) & 1ull)) pa.rest_pos[k++] = pa.rest_pos[j];
+            pa.nrest = (uint8_t)k;
+        }
+    }
+    return pa;
+}
+
 // Relative HBM time of a pass whose tiles are read and written in runs of
 // `run_bytes` contiguous bytes (one MI355X, 30-qubit passes: 64 B runs cost
 // 1.37x, 256 B runs 0.92x the time of 128 B runs).

@@ -165,4 +165,11 @@ struct Plan {
 // Builds the pass plan.  Returns an empty string on success, else the error.
 std::string build_plan(const Program& prog, const PlanOptions& opt, Plan& out);
 
+// Launch arguments of pass i of a run that starts from |0...0> without an
+// init kernel: pass 0 makes its tile in registers; with `support`, a pass
+// reads no amplitude known to be zero (Plan::pass_zero) and, before the
+// first remap pass and unless it is the last pass, launches only the tiles
+// whose rest qubits are all outside that set (fewer rest qubits).
+PassArgs zero_start_pass(const Plan& plan, size_t i, bool support);
+
 }  // namespace kate

@@ -186,18 +186,22 @@ int launch_request(kate_server* s, Request& r)
         SV(hipMemcpyAsync(base + r.o_raw, r.plan.raw.data(), r.plan.raw.size() * sizeof(uint32_t),
                           hipMemcpyHostToDevice, st));
     SV(hipMemcpyAsync(base + r.o_masks, r.masks.data(), r.masks.size() * sizeof(uint64_t), hipMemcpyHostToDevice, st));
-    int k;
+    int k = 0;
     if (!r.product.empty()) {
         SV(hipMemcpyAsync(base + r.o_factors, r.product.data(), r.product.size() * sizeof(double),
                           hipMemcpyHostToDevice, st));
         k = kate::launch_init_product(s->opt.dtype, state, r.plan.nq_int, (const double*)(base + r.o_factors), st);
-    } else {
+    } else if (r.plan.passes.empty()) {
         k = kate::launch_init_basis(s->opt.dtype, state, r.plan.nq_int, 0, st);
     }
     if (k) return k;
+    // from |0...0> the first pass makes the state and the passes follow its
+    // support (no init kernel); a product state is read as written
+    const bool from_zero = r.product.empty();
     kate::DevPlan dp{(const kate::MicroStep*)(base + r.o_micro), (const kate::MicroOp*)(base + r.o_ops),
                      base + r.o_coef, (const uint32_t*)(base + r.o_raw)};
-    for (const kate::PassArgs& pa : r.plan.passes) {
+    for (size_t i = 0; i < r.plan.passes.size(); i++) {
+        const kate::PassArgs pa = from_zero ? kate::zero_start_pass(r.plan, i, s->po.support) : r.plan.passes[i];
         k = kate::launch_pass(s->opt.dtype, r.plan.tile_bits, state, r.plan.nq_int, pa, dp, st);
         if (k) return k;
     }

@@ -205,37 +205,15 @@ void release(kate_sim* s)
     delete s;
 }
 
-// Runs every pass.  With `basis` set, the state is taken to be the basis
-// state |basis_index> without being read: the first pass makes its tiles in
-// registers (PassArgs::synth_basis), so the caller skips the init kernel.
-// From |0...0> the passes also follow the plan's support (Plan::pass_zero):
-// a pass reads no amplitude that is known to be zero, and a pass before the
-// first remap that is not the last one launches only the tiles whose rest
-// qubits are all outside the zero set -- every other tile holds zeros in
-// and out, and no later pass reads it before a pass has written it (a rest
-// qubit keeps its zero status through the pass; the last pass writes every
-// tile, so the final state is whole).
-int run_passes(kate_sim* s, bool basis = false, uint64_t basis_index = 0)
+// Runs every pass on the current state, or, with `from_zero`, on |0...0>
+// without an init kernel and without reading it (kate::zero_start_pass:
+// the first pass makes its tile in registers and, unless the support
+// skipping is off, the passes follow the support of the state).
+int run_passes(kate_sim* s, bool from_zero = false)
 {
     kate::DevPlan dp{s->d_micro, s->d_ops, s->d_coef, s->d_raw};
-    const bool support = basis && basis_index == 0 && s->support;
-    const size_t np = s->plan.passes.size();
-    for (size_t i = 0; i < np; i++) {
-        kate::PassArgs pa = s->plan.passes[i];
-        if (basis && i == 0) {
-            pa.synth_basis = 1;
-            pa.synth_index = basis_index;
-        }
-        if (support && i < s->plan.pass_zero.size()) {
-            const uint64_t z = s->plan.pass_zero[i];
-            if (i > 0) pa.zero_mask = z;
-            if ((int)i < s->plan.support_passes && i + 1 < np) {
-                int k = 0;
-                for (int j = 0; j < pa.nrest; j++)
-                    if (!((z >> pa.rest_pos[j]) & 1ull)) pa.rest_pos[k++] = pa.rest_pos[j];
-                pa.nrest = (uint8_t)k;
-            }
-        }
+    for (size_t i = 0; i < s->plan.passes.size(); i++) {
+        const kate::PassArgs pa = from_zero ? kate::zero_start_pass(s->plan, i, s->support) : s->plan.passes[i];
         int e = kate::launch_pass(s->dtype, s->plan.tile_bits, s->d_state, s->nq_int, pa, dp, s->stream);
         if (e) return fail(KATE_ERR_HIP, std::string("pass launch: ") + hipGetErrorString((hipError_t)e));
     }
@@ -605,7 +583,7 @@ int kate_sim_execute(kate_sim* s, const double* tp_in, double* tp_out)
     else if (!fused_init) rc = kate::launch_init_basis(s->dtype, s->d_state, s->nq_int, 0, s->stream);
     if (rc) return rc > 0 ? fail(KATE_ERR_HIP, "init launch failed") : rc;
     KATE_HIP(hipEventRecord(s->ev[1], s->stream));
-    rc = run_passes(s, fused_init, 0);
+    rc = run_passes(s, fused_init);
     if (rc) return rc;
     KATE_HIP(hipEventRecord(s->ev[2], s->stream));
     std::vector<double> tmp(2 * (size_t)s->nq);
