@@ -112,7 +112,11 @@ enum {
     /* execute() from |0...0>: read and launch every tile instead of only
        those that can hold non-zero amplitudes (qubits no gate has touched
        yet keep every amplitude with their bit set at exactly zero) */
-    KATE_PLAN_NO_SUPPORT = 4
+    KATE_PLAN_NO_SUPPORT = 4,
+    /* execute(): run the trailing passes that only permute amplitudes
+       (CNOT layers at the end of a circuit) instead of measuring through
+       them and running them when the state is next read */
+    KATE_PLAN_NO_DEFER = 8
 };
 
 /* Wall-clock split of the last execute, as Kate's execute(true) prints it

@@ -40,11 +40,12 @@ class Options(C.Structure):
                 ("micro_bits", C.c_int32), ("plan_flags", C.c_int32)]
 
 
-PLAN_NO_EULER, PLAN_NO_DIRECT, PLAN_NO_SUPPORT = 1, 2, 4
+PLAN_NO_EULER, PLAN_NO_DIRECT, PLAN_NO_SUPPORT, PLAN_NO_DEFER = 1, 2, 4, 8
 
 
-def _plan_flags(euler: bool, direct: bool, support: bool = True) -> int:
-    return (0 if euler else PLAN_NO_EULER) | (0 if direct else PLAN_NO_DIRECT) | (0 if support else PLAN_NO_SUPPORT)
+def _plan_flags(euler: bool, direct: bool, support: bool = True, defer: bool = True) -> int:
+    return ((0 if euler else PLAN_NO_EULER) | (0 if direct else PLAN_NO_DIRECT) | (0 if support else PLAN_NO_SUPPORT)
+            | (0 if defer else PLAN_NO_DEFER))
 
 
 class Timings(C.Structure):
@@ -304,12 +305,12 @@ class Simulator:
 
     def __init__(self, circuit: Circuit, dtype: str = "c128", tile_qubits: int = 0, low_qubits: int = 0,
                  device: int = 0, mfma: int = 0, micro_bits: int = 0, euler: bool = True, direct: bool = True,
-                 support: bool = True):
+                 support: bool = True, defer: bool = True):
         self.n = circuit.n
         self.dtype = {"c128": KATE_C128, "c64": KATE_C64}[dtype]
         self.amp = np.complex128 if self.dtype == KATE_C128 else np.complex64
         opt = Options(self.dtype, tile_qubits, low_qubits, mfma, 0, device, micro_bits,
-                      _plan_flags(euler, direct, support))
+                      _plan_flags(euler, direct, support, defer))
         h = VP()
         _check(lib().kate_sim_create_from_circuit(circuit._h, C.byref(opt), C.byref(h)), "kate_sim_create")
         self._h = h

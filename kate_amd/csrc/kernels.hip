@@ -998,16 +998,40 @@ __global__ __launch_bounds__(256) void k_scale(Cx<R>* __restrict__ st, uint64_t 
 
 // Level 0 of the amplitude sums: a block reduces 4096 amplitudes (tile index
 // it*256 + tid) to one record {total, S1[bit 0..11]}.
-template <typename R>
-__global__ __launch_bounds__(256) void k_sums0(const Cx<R>* __restrict__ st, double* __restrict__ rec)
+// With GATHER the amplitude of index i is read at offset M(i), M linear over
+// GF(2) with columns gm.col[] (a state whose last permutation passes have
+// not run yet: M maps each final index to where its amplitude still is).
+struct GatherMap {
+    uint64_t col[kMaxQubits];
+};
+
+template <typename R, bool GATHER>
+__global__ __launch_bounds__(256) void k_sums0(const Cx<R>* __restrict__ st, double* __restrict__ rec, GatherMap gm)
 {
     __shared__ double lds[4];
     const uint32_t tid = threadIdx.x;
     const uint64_t base = (uint64_t)blockIdx.x << 12;
+    uint64_t gbase = 0;
+    if constexpr (GATHER) {
+        const uint32_t blk = blockIdx.x;
+#pragma unroll
+        for (int j = 0; j < kMaxQubits - 12; j++)
+            if ((blk >> j) & 1u) gbase ^= gm.col[12 + j];
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if ((tid >> j) & 1u) gbase ^= gm.col[j];
+    }
     double tr = 0, ti = 0, hr[4] = {0, 0, 0, 0}, hi[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int it = 0; it < 16; it++) {
-        const Cx<R> a = st[base + ((uint64_t)it << 8) + tid];
+        uint64_t off = base + ((uint64_t)it << 8) + tid;
+        if constexpr (GATHER) {
+            off = gbase;
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if ((it >> j) & 1) off ^= gm.col[8 + j];
+        }
+        const Cx<R> a = st[off];
         tr += (double)a.re;
         ti += (double)a.im;
 #pragma unroll
@@ -1343,17 +1367,37 @@ uint64_t reduce_scratch_doubles(int nq_int)
     return chunks * 26 * 2 + chunks * 16 * 2 + 4096;
 }
 
-int launch_amp_sums(int dtype, const void* state, int nq_int, double* d_scratch, double* d_out, void* stream)
+int launch_amp_sums(int dtype, const void* state, int nq_int, double* d_scratch, double* d_out, void* stream,
+                    const uint64_t* gather_cols)
 {
     hipStream_t s = (hipStream_t)stream;
-    if (nq_int < 12) return (int)hipErrorInvalidValue;
+    if (nq_int < 12 || nq_int > kMaxQubits) return (int)hipErrorInvalidValue;
     uint64_t count = 1ull << (nq_int - 12);
     double* a = count == 1 ? d_out : d_scratch;
     double* b = d_scratch + count * 26 + 64;
-    if (dtype == 0)
-        hipLaunchKernelGGL(k_sums0<double>, dim3((unsigned)count), dim3(256), 0, s, (const Cx<double>*)state, a);
-    else
-        hipLaunchKernelGGL(k_sums0<float>, dim3((unsigned)count), dim3(256), 0, s, (const Cx<float>*)state, a);
+    GatherMap gm{};
+    if (gather_cols) {
+        const uint64_t lim = nq_int >= 64 ? ~0ull : ((1ull << nq_int) - 1ull);
+        for (int q = 0; q < nq_int; q++) {
+            if (gather_cols[q] & ~lim) return (int)hipErrorInvalidValue;  // offsets stay inside the state
+            gm.col[q] = gather_cols[q];
+        }
+    }
+    if (dtype == 0) {
+        if (gather_cols)
+            hipLaunchKernelGGL((k_sums0<double, true>), dim3((unsigned)count), dim3(256), 0, s,
+                               (const Cx<double>*)state, a, gm);
+        else
+            hipLaunchKernelGGL((k_sums0<double, false>), dim3((unsigned)count), dim3(256), 0, s,
+                               (const Cx<double>*)state, a, gm);
+    } else {
+        if (gather_cols)
+            hipLaunchKernelGGL((k_sums0<float, true>), dim3((unsigned)count), dim3(256), 0, s,
+                               (const Cx<float>*)state, a, gm);
+        else
+            hipLaunchKernelGGL((k_sums0<float, false>), dim3((unsigned)count), dim3(256), 0, s,
+                               (const Cx<float>*)state, a, gm);
+    }
     int K = 12;
     while (count > 1) {
         int C = 0;

@@ -70,7 +70,11 @@ uint64_t reduce_scratch_doubles(int nq_int);
 // Amplitude sums: out[0..1] = sum of all amplitudes, out[2+2q..] = sum of the
 // amplitudes whose bit q is 1, for q < nq_int (complex128, FP64 accumulation,
 // fixed summation order).
-int launch_amp_sums(int dtype, const void* state, int nq_int, double* d_scratch, double* d_out, void* stream);
+// gather_cols (nullable, nq_int entries): read index i at the GF(2)-linear
+// offset XOR_{q in i} gather_cols[q] (a state whose trailing permutation
+// passes have not run).
+int launch_amp_sums(int dtype, const void* state, int nq_int, double* d_scratch, double* d_out, void* stream,
+                    const uint64_t* gather_cols = nullptr);
 // out[j] = sum_x |a_x|^2 (-1)^popcount(x & masks[j]) for j < count (FP64).
 int launch_zsums(int dtype, const void* state, int nq_int, const uint64_t* d_masks, int count, double* d_scratch,
                  double* d_out, void* stream);

@@ -982,7 +982,7 @@ struct Builder {
 
 }  // namespace
 
-PassArgs zero_start_pass(const Plan& plan, size_t i, bool support)
+PassArgs zero_start_pass(const Plan& plan, size_t i, bool support, size_t n_run)
 {
     PassArgs pa = plan.passes[i];
     if (i == 0) {
@@ -995,7 +995,7 @@ PassArgs zero_start_pass(const Plan& plan, size_t i, bool support)
         // status through a pass, and the last pass writes every tile.
         const uint64_t z = plan.pass_zero[i];
         if (i > 0) pa.zero_mask = z;
-        if ((int)i < plan.support_passes && i + 1 < plan.passes.size()) {
+        if ((int)i < plan.support_passes && i + 1 < n_run) {
             int k = 0;
             for (int j = 0; j < pa.nrest; j++)
                 if (!((z >> pa.rest_pos[j]) & 1ull)) pa.rest_pos[k++] = pa.rest_pos[j];
@@ -1003,6 +1003,79 @@ PassArgs zero_start_pass(const Plan& plan, size_t i, bool support)
         }
     }
     return pa;
+}
+
+// Inverse of an invertible GF(2) matrix given by its columns (n <= 64):
+// inv[j] = the vector whose image is 2^j.  False if it is singular.
+static bool gf2_inverse(std::vector<uint64_t> a, std::vector<uint64_t>& inv)
+{
+    const size_t n = a.size();
+    inv.assign(n, 0);
+    for (size_t b = 0; b < n; b++) inv[b] = 1ull << b;
+    for (size_t j = 0; j < n; j++) {
+        size_t piv = n;
+        for (size_t b = j; b < n; b++)
+            if ((a[b] >> j) & 1ull) {
+                piv = b;
+                break;
+            }
+        if (piv == n) return false;
+        std::swap(a[j], a[piv]);
+        std::swap(inv[j], inv[piv]);
+        for (size_t b = 0; b < n; b++)
+            if (b != j && ((a[b] >> j) & 1ull)) {
+                a[b] ^= a[j];
+                inv[b] ^= inv[j];
+            }
+    }
+    return true;
+}
+
+// The trailing pure-permutation passes as one linear map of state offsets
+// (Plan::trail_begin, Plan::trail_inv).  A pass without micro-steps moves
+// the amplitude at tile index u (rest offset r) to r ^ store(L^-1(u)), L its
+// final layout.
+static void plan_trailing(Plan& plan)
+{
+    const int n = plan.nq_int;
+    size_t b = plan.passes.size();
+    while (b > 0 && plan.passes[b - 1].micro_count == 0) b--;
+    plan.trail_begin = b;
+    plan.trail_inv.clear();
+    if (b == plan.passes.size() || n > 64) {
+        plan.trail_begin = plan.passes.size();
+        return;
+    }
+    std::vector<uint64_t> img((size_t)n);
+    for (int q = 0; q < n; q++) img[(size_t)q] = 1ull << q;
+    for (size_t k = b; k < plan.passes.size(); k++) {
+        const PassArgs& pa = plan.passes[k];
+        const int t = pa.tile_bits;
+        std::vector<uint64_t> cols((size_t)t), linv;
+        for (int j = 0; j < t; j++) cols[(size_t)j] = pa.store_cols[j];
+        if (!gf2_inverse(cols, linv)) {
+            plan.trail_begin = plan.passes.size();
+            return;
+        }
+        for (int q = 0; q < n; q++) {
+            uint64_t v = img[(size_t)q], u = 0;
+            for (int j = 0; j < t; j++)
+                if ((v >> pa.load_pos[j]) & 1ull) {
+                    u |= 1ull << j;
+                    v &= ~(1ull << pa.load_pos[j]);
+                }
+            uint64_t y = 0;
+            for (int j = 0; j < t; j++)
+                if ((u >> j) & 1ull) y ^= linv[(size_t)j];
+            for (int j = 0; j < t; j++)
+                if ((y >> j) & 1ull) v ^= 1ull << pa.store_pos[j];
+            img[(size_t)q] = v;
+        }
+    }
+    if (!gf2_inverse(img, plan.trail_inv)) {
+        plan.trail_begin = plan.passes.size();
+        plan.trail_inv.clear();
+    }
 }
 
 // Relative HBM time of a pass whose tiles are read and written in runs of
@@ -1085,6 +1158,7 @@ std::string build_plan(const Program& prog, const PlanOptions& opt, Plan& plan)
         std::string e = b.gate_passes(seg);
         if (!e.empty()) return e;
     }
+    plan_trailing(plan);
     return "";
 }
 

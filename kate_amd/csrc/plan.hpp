@@ -138,6 +138,7 @@ struct PlanOptions {
     bool direct = true;      // first / last SK_REG step of a pass loads / stores HBM directly
     bool auto_low = false;   // build the plan for several low_qubits and keep the cheapest
     bool support = true;     // runs from |0...0> skip tiles known to be zero (used by the simulator)
+    bool defer = true;       // execute() measures through the trailing permutation passes (simulator)
 };
 
 struct Plan {
@@ -159,6 +160,13 @@ struct Plan {
     // tiles whose non-tile qubits meet that set.
     std::vector<uint64_t> pass_zero;
     int support_passes = 0;
+    // Trailing passes [trail_begin, passes.size()) only permute amplitudes
+    // (lazy-CNOT layouts or remaps, no micro-steps): together a GF(2)-linear
+    // map A of state offsets.  trail_inv[q] = A^-1(2^q), the offset before
+    // those passes of the amplitude that ends at offset 2^q; a measurement
+    // can read the state through it instead of running them.
+    size_t trail_begin = 0;
+    std::vector<uint64_t> trail_inv;
     std::string describe(bool verbose = false) const;
 };
 
@@ -169,7 +177,8 @@ std::string build_plan(const Program& prog, const PlanOptions& opt, Plan& out);
 // init kernel: pass 0 makes its tile in registers; with `support`, a pass
 // reads no amplitude known to be zero (Plan::pass_zero) and, before the
 // first remap pass and unless it is the last pass, launches only the tiles
-// whose rest qubits are all outside that set (fewer rest qubits).
-PassArgs zero_start_pass(const Plan& plan, size_t i, bool support);
+// whose rest qubits are all outside that set (fewer rest qubits).  n_run is
+// the number of passes the run launches (the last of them writes every tile).
+PassArgs zero_start_pass(const Plan& plan, size_t i, bool support, size_t n_run);
 
 }  // namespace kate

@@ -201,7 +201,7 @@ int launch_request(kate_server* s, Request& r)
     kate::DevPlan dp{(const kate::MicroStep*)(base + r.o_micro), (const kate::MicroOp*)(base + r.o_ops),
                      base + r.o_coef, (const uint32_t*)(base + r.o_raw)};
     for (size_t i = 0; i < r.plan.passes.size(); i++) {
-        const kate::PassArgs pa = from_zero ? kate::zero_start_pass(r.plan, i, s->po.support) : r.plan.passes[i];
+        const kate::PassArgs pa = from_zero ? kate::zero_start_pass(r.plan, i, s->po.support, r.plan.passes.size()) : r.plan.passes[i];
         k = kate::launch_pass(s->opt.dtype, r.plan.tile_bits, state, r.plan.nq_int, pa, dp, st);
         if (k) return k;
     }

@@ -170,6 +170,8 @@ struct kate_sim {
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     kate_timings last{};
     bool support = true;  // execute() from |0...0> skips tiles known to be zero
+    bool defer = true;    // execute() measures through the trailing permutation passes
+    bool pending = false; // those passes have not run on the state yet
 };
 
 namespace {
@@ -209,15 +211,27 @@ void release(kate_sim* s)
 // without an init kernel and without reading it (kate::zero_start_pass:
 // the first pass makes its tile in registers and, unless the support
 // skipping is off, the passes follow the support of the state).
-int run_passes(kate_sim* s, bool from_zero = false)
+// Passes [begin, end) -- all of them by default.
+int run_passes(kate_sim* s, bool from_zero = false, size_t begin = 0, size_t end = (size_t)-1)
 {
     kate::DevPlan dp{s->d_micro, s->d_ops, s->d_coef, s->d_raw};
-    for (size_t i = 0; i < s->plan.passes.size(); i++) {
-        const kate::PassArgs pa = from_zero ? kate::zero_start_pass(s->plan, i, s->support) : s->plan.passes[i];
+    end = std::min(end, s->plan.passes.size());
+    for (size_t i = begin; i < end; i++) {
+        const kate::PassArgs pa = from_zero ? kate::zero_start_pass(s->plan, i, s->support, end) : s->plan.passes[i];
         int e = kate::launch_pass(s->dtype, s->plan.tile_bits, s->d_state, s->nq_int, pa, dp, s->stream);
         if (e) return fail(KATE_ERR_HIP, std::string("pass launch: ") + hipGetErrorString((hipError_t)e));
     }
     return KATE_OK;
+}
+
+// execute() can leave the trailing permutation passes unrun (its
+// measurement reads through them); every other use of the state runs them
+// first.
+int ensure_final(kate_sim* s)
+{
+    if (!s->pending) return KATE_OK;
+    s->pending = false;
+    return run_passes(s, false, s->plan.trail_begin);
 }
 
 int init_product(kate_sim* s, const double* tp)
@@ -249,7 +263,10 @@ int init_product(kate_sim* s, const double* tp)
 // Kate's proba_state from the amplitude sums (HIPcode/simulator.hpp:519-536).
 int measure(kate_sim* s, double* out)
 {
-    int e = kate::launch_amp_sums(s->dtype, s->d_state, s->nq_int, s->d_scratch, s->d_out, s->stream);
+    // with the trailing passes pending, amplitude i is still at offset
+    // trail_inv(i)
+    const uint64_t* gather = s->pending ? s->plan.trail_inv.data() : nullptr;
+    int e = kate::launch_amp_sums(s->dtype, s->d_state, s->nq_int, s->d_scratch, s->d_out, s->stream, gather);
     if (e) return fail(KATE_ERR_HIP, std::string("measure launch: ") + hipGetErrorString((hipError_t)e));
     std::vector<double> sums(2 + 2 * (size_t)s->nq_int);
     KATE_HIP(hipMemcpyAsync(sums.data(), s->d_out, sums.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
@@ -326,8 +343,9 @@ std::string plan_options(const kate_options& opt, kate::PlanOptions& po)
     po.mfma_bf16 = opt.use_mfma == 3;
     po.micro_bits = opt.micro_bits ? opt.micro_bits : 4;
     po.amp_bytes = opt.dtype == KATE_C128 ? 16 : 8;
-    if (opt.plan_flags & ~(KATE_PLAN_NO_EULER | KATE_PLAN_NO_DIRECT | KATE_PLAN_NO_SUPPORT))
+    if (opt.plan_flags & ~(KATE_PLAN_NO_EULER | KATE_PLAN_NO_DIRECT | KATE_PLAN_NO_SUPPORT | KATE_PLAN_NO_DEFER))
         return "unknown plan_flags bits";
+    po.defer = !(opt.plan_flags & KATE_PLAN_NO_DEFER);
     po.euler = !(opt.plan_flags & KATE_PLAN_NO_EULER);
     po.direct = !(opt.plan_flags & KATE_PLAN_NO_DIRECT);
     po.support = !(opt.plan_flags & KATE_PLAN_NO_SUPPORT);
@@ -450,6 +468,7 @@ int kate_sim_create(const kate_program* program, const kate_options* options, ka
         return fail(KATE_ERR_ARG, err);
     }
     s->support = po.support;
+    s->defer = po.defer;
     err = kate::build_plan(s->prog, po, s->plan);
     if (!err.empty()) {
         delete s;
@@ -513,6 +532,7 @@ int kate_sim_init_basis(kate_sim* s, uint64_t index)
     if (!s) return fail(KATE_ERR_ARG, "null simulator");
     if (index >= (1ull << s->nq)) return fail(KATE_ERR_ARG, "basis index out of range");
     KATE_HIP(hipSetDevice(s->device));
+    s->pending = false;  // the whole state is replaced
     int e = kate::launch_init_basis(s->dtype, s->d_state, s->nq_int, index, s->stream);
     if (e) return fail(KATE_ERR_HIP, std::string("init launch: ") + hipGetErrorString((hipError_t)e));
     return KATE_OK;
@@ -522,6 +542,7 @@ int kate_sim_init_product(kate_sim* s, const double* tp)
 {
     if (!s || !tp) return fail(KATE_ERR_ARG, "null argument");
     KATE_HIP(hipSetDevice(s->device));
+    s->pending = false;  // the whole state is replaced
     return init_product(s, tp);
 }
 
@@ -529,6 +550,8 @@ int kate_sim_run(kate_sim* s)
 {
     if (!s) return fail(KATE_ERR_ARG, "null simulator");
     KATE_HIP(hipSetDevice(s->device));
+    int rc = ensure_final(s);
+    if (rc) return rc;
     return run_passes(s);
 }
 
@@ -536,6 +559,7 @@ int kate_sim_run_timed(kate_sim* s, float* pass_ms, int32_t* pass_is_remap, int3
 {
     if (!s) return fail(KATE_ERR_ARG, "null simulator");
     KATE_HIP(hipSetDevice(s->device));
+    if (int rc0 = ensure_final(s)) return rc0;
     const size_t np = s->plan.passes.size();
     if (npasses) *npasses = (int32_t)np;
     std::vector<hipEvent_t> ev(np + 1, nullptr);
@@ -567,7 +591,7 @@ int kate_sim_measure_proba(kate_sim* s, double* tp)
 {
     if (!s || !tp) return fail(KATE_ERR_ARG, "null argument");
     KATE_HIP(hipSetDevice(s->device));
-    return measure(s, tp);
+    return measure(s, tp);  // reads through pending trailing passes
 }
 
 int kate_sim_execute(kate_sim* s, const double* tp_in, double* tp_out)
@@ -577,14 +601,19 @@ int kate_sim_execute(kate_sim* s, const double* tp_in, double* tp_out)
     KATE_HIP(hipEventRecord(s->ev[0], s->stream));
     // |0...0> is made by the first pass itself (no init pass over HBM) unless
     // there is no pass to make it
+    s->pending = false;  // the state is made afresh
     const bool fused_init = !tp_in && !s->plan.passes.empty();
     int rc = KATE_OK;
     if (tp_in) rc = init_product(s, tp_in);
     else if (!fused_init) rc = kate::launch_init_basis(s->dtype, s->d_state, s->nq_int, 0, s->stream);
     if (rc) return rc > 0 ? fail(KATE_ERR_HIP, "init launch failed") : rc;
     KATE_HIP(hipEventRecord(s->ev[1], s->stream));
-    rc = run_passes(s, fused_init);
+    // trailing permutation passes are left for whoever reads the state next;
+    // the measurement below reads through them
+    const size_t n_run = (s->defer && s->plan.trail_begin > 0) ? s->plan.trail_begin : s->plan.passes.size();
+    rc = run_passes(s, fused_init, 0, n_run);
     if (rc) return rc;
+    s->pending = n_run < s->plan.passes.size();
     KATE_HIP(hipEventRecord(s->ev[2], s->stream));
     std::vector<double> tmp(2 * (size_t)s->nq);
     rc = measure(s, tp_out ? tp_out : tmp.data());
@@ -605,6 +634,7 @@ int kate_sim_expect_z(kate_sim* s, const uint64_t* masks, int32_t count, double*
 {
     if (!s || (count > 0 && (!masks || !out)) || count < 0) return fail(KATE_ERR_ARG, "bad arguments");
     KATE_HIP(hipSetDevice(s->device));
+    if (int rc0 = ensure_final(s)) return rc0;
     std::vector<uint64_t> vm;
     vm.push_back(0);  // norm
     for (int j = 0; j < count; j++) {
@@ -622,6 +652,7 @@ int kate_sim_norm(kate_sim* s, double* out)
 {
     if (!s || !out) return fail(KATE_ERR_ARG, "null argument");
     KATE_HIP(hipSetDevice(s->device));
+    if (int rc0 = ensure_final(s)) return rc0;
     std::vector<double> r;
     int rc = zsums(s, std::vector<uint64_t>{0}, r);
     if (rc) return rc;
@@ -646,6 +677,7 @@ int kate_sim_read_state(kate_sim* s, void* host, uint64_t bytes, int32_t real_or
     const uint64_t amps = 1ull << s->nq;
     if (bytes != amps * s->amp_bytes) return fail(KATE_ERR_ARG, "buffer size must be 2^n amplitudes");
     KATE_HIP(hipSetDevice(s->device));
+    if (int rc0 = ensure_final(s)) return rc0;
     bool identity = true;
     for (int p = 0; p < s->nq; p++)
         if (s->prog.final_inverse_permutation[(size_t)p] != p) identity = false;
@@ -673,6 +705,7 @@ int kate_sim_write_state(kate_sim* s, const void* host, uint64_t bytes)
     const uint64_t amps = 1ull << s->nq;
     if (bytes != amps * s->amp_bytes) return fail(KATE_ERR_ARG, "buffer size must be 2^n amplitudes");
     KATE_HIP(hipSetDevice(s->device));
+    s->pending = false;  // the whole state is replaced
     if (s->nq_int > s->nq) {
         int e = kate::launch_init_basis(s->dtype, s->d_state, s->nq_int, 1ull << s->nq_int, s->stream);  // zero fill
         if (e) return fail(KATE_ERR_HIP, "zero fill launch failed");
@@ -690,6 +723,8 @@ int kate_sim_overlap(kate_sim* a, kate_sim* b, double* out5)
     if (a->prog.final_inverse_permutation != b->prog.final_inverse_permutation)
         return fail(KATE_ERR_ARG, "overlap needs the same virtual qubit order in both states");
     KATE_HIP(hipSetDevice(a->device));
+    if (int rc0 = ensure_final(a)) return rc0;
+    if (int rc0 = ensure_final(b)) return rc0;
     KATE_HIP(hipStreamSynchronize(b->stream));
     int e = kate::launch_overlap(a->dtype, a->d_state, b->dtype, b->d_state, a->nq_int, a->d_scratch, a->d_out,
                                  a->stream);
@@ -702,6 +737,8 @@ int kate_sim_overlap(kate_sim* a, kate_sim* b, double* out5)
 int kate_sim_device_state(kate_sim* s, void** dptr)
 {
     if (!s || !dptr) return fail(KATE_ERR_ARG, "null argument");
+    KATE_HIP(hipSetDevice(s->device));
+    if (int rc0 = ensure_final(s)) return rc0;
     *dptr = s->d_state;
     return KATE_OK;
 }

@@ -113,3 +113,31 @@ def test_support_skipping_at_30_qubits(gpu):
         b.execute()
         _, na, nb, mx = a.compare(b)
     assert mx == 0.0 and abs(na - 1.0) <= 1e-12
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_execute_measures_through_trailing_permutations(gpu, dtype):
+    """A circuit that ends with CNOT layers leaves its last passes as pure
+    permutations of the state: execute() measures through them (the sums
+    read amplitude i where it still is) and runs them when the state is
+    next read.  The proba_state must equal the one measured on the finished
+    state, and the finished state the one of a run without the deferral."""
+    say("deferred trailing passes", dtype)
+    for n, gates in ((20, W.layered(20, 3, 41)), (22, W.layered(22, 3, 42)),
+                     (24, W.haar_layers(24, 1, 43) + W.layered(24, 2, 44))):
+        _, _, text = K.Circuit(n, gates).plan(dtype)
+        assert " micro=0 " in text.splitlines()[-1] + " "  # the last pass only permutes
+        with K.Simulator(K.Circuit(n, gates), dtype=dtype) as a, \
+                K.Simulator(K.Circuit(n, gates), dtype=dtype, defer=False) as b:
+            pa = a.execute()
+            pb = b.execute()
+            sa = a.state()  # runs the deferred passes
+            pa2 = a.measure_proba()
+            sb = b.state()
+        tol = 1e-12 if dtype == "c128" else 1e-5
+        assert np.max(np.abs(pa - pb)) <= tol and np.max(np.abs(pa - pa2)) <= tol
+        assert np.array_equal(sa, sb)
+        if dtype == "c128" and n <= 20:
+            ref = O.simulate(n, gates)
+            assert np.max(np.abs(sa - ref)) <= 1e-12
+            assert np.max(np.abs(pa - O.measure(n, ref))) <= 1e-9
