@@ -10,9 +10,10 @@ sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X); runs on the GPU box")
+    config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X)")
     # Build only what is missing: a checkout that already holds the built
-    # libraries (the GPU box) never rebuilds.
+    # libraries (e.g. a copy made on a machine with the toolchain) never
+    # rebuilds.
     if not os.path.exists(os.path.join(ROOT, "oracle", "liboracle.so")):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "liboracle.so"], check=True)
     if not os.path.exists(os.path.join(ROOT, "kate_amd", "libkate_hip.so")):
