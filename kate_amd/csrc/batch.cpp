@@ -150,6 +150,9 @@ int kate_batch_create(const kate_program* programs, int32_t count, const kate_op
     std::vector<double> coef;
     std::vector<uint32_t> raw;
     std::vector<uint32_t> pass_base, npass, tiles;
+    // per merged pass: the qubits still exactly |0> when it starts (every
+    // state starts at |0...0>), and its rest positions
+    std::vector<uint64_t> pass_zero;
     for (int32_t c = 0; c < count; c++) {
         kate::Program prog;
         err = kate::program_from_abi(&programs[c], prog);
@@ -177,9 +180,11 @@ int kate_batch_create(const kate_program* programs, int32_t count, const kate_op
         pass_base.push_back((uint32_t)passes.size());
         npass.push_back((uint32_t)plan.passes.size());
         tiles.push_back(1u << (plan.nq_int - plan.tile_bits));
-        for (kate::PassArgs pa : plan.passes) {
+        for (size_t i = 0; i < plan.passes.size(); i++) {
+            kate::PassArgs pa = plan.passes[i];
             pa.micro_begin += mb;
             passes.push_back(pa);
+            pass_zero.push_back(i < plan.pass_zero.size() && (int)i < plan.support_passes ? plan.pass_zero[i] : 0ull);
         }
         circ_off.push_back(b->total_amps);
         b->total_amps += 1ull << plan.nq_int;
@@ -189,21 +194,32 @@ int kate_batch_create(const kate_program* programs, int32_t count, const kate_op
         b->zlen += prog.nqubits;
     }
     b->npasses = (uint32_t)passes.size();
-    // tasks by pass level, then circuit, then tile
+    // tasks by pass level, then circuit, then tile.  Every state starts at
+    // |0...0> (written by the fill kernels), so a tile whose rest qubits
+    // include one that no gate has touched yet is zero before and after
+    // the pass: its task is left out (the memory already holds its zeros).
     std::vector<kate::BatchTask> tasks;
+    std::vector<uint32_t> ntask_of_pass(passes.size(), 0);
     const uint32_t maxp = *std::max_element(npass.begin(), npass.end());
     for (uint32_t p = 0; p < maxp; p++)
         for (uint32_t c = 0; c < (uint32_t)count; c++) {
             if (p >= npass[c]) continue;
+            const uint32_t gp = pass_base[c] + p;
+            const kate::PassArgs& pa = passes[gp];
+            uint32_t zero_rest = 0;  // tile-index bits of rest qubits still |0>
+            for (int j = 0; j < pa.nrest; j++)
+                if ((pass_zero[gp] >> pa.rest_pos[j]) & 1ull) zero_rest |= 1u << j;
             for (uint32_t t = 0; t < tiles[c]; t++) {
+                if (t & zero_rest) continue;
                 kate::BatchTask k{};
                 k.circuit = c;
-                k.pass = pass_base[c] + p;
+                k.pass = gp;
                 k.tile = t;
-                k.wait_pass = p ? pass_base[c] + p - 1 : 0xFFFFFFFFu;
-                k.wait_count = p ? tiles[c] : 0;
+                k.wait_pass = p ? gp - 1 : 0xFFFFFFFFu;
+                k.wait_count = p ? ntask_of_pass[gp - 1] : 0;
                 k.signal = p + 1 < npass[c] ? 1u : 0u;
                 tasks.push_back(k);
+                ntask_of_pass[gp]++;
             }
         }
     b->ntasks = (uint32_t)tasks.size();
