@@ -191,3 +191,19 @@ def test_direct_plans_take_the_direct_path():
     lines = [ln for ln in text.splitlines() if ln.startswith("pass")]
     assert sum("direct=first" in ln for ln in lines) >= len(lines) // 2
     assert sum("last" in ln for ln in lines) >= len(lines) // 3
+
+
+def test_plan_flags_validation_and_trailing_passes():
+    """Unknown plan_flags bits are refused; the layered circuits end with
+    passes that only permute amplitudes (the CNOT ladder), the ones that
+    execute() measures through."""
+    import ctypes as C
+    n = 22
+    circ = K.Circuit(n, W.layered(n, 3, 42))
+    opt = K.Options(K.KATE_C128, 0, 0, 0, 0, 0, 0, 16)
+    p, r = C.c_int32(), C.c_int32()
+    buf = C.create_string_buffer(1 << 16)
+    rc = K.lib().kate_circuit_plan(circ._h, C.byref(opt), C.byref(p), C.byref(r), buf, 1 << 16)
+    assert rc != 0 and "plan_flags" in K.lib().kate_last_error().decode()
+    lines = [ln for ln in circ.plan("c128")[2].splitlines() if ln.startswith("pass")]
+    assert " micro=0 " in lines[-1] + " "
