@@ -1008,7 +1008,6 @@ struct GatherMap {
 template <typename R, bool GATHER>
 __global__ __launch_bounds__(256) void k_sums0(const Cx<R>* __restrict__ st, double* __restrict__ rec, GatherMap gm)
 {
-    __shared__ double lds[4];
     const uint32_t tid = threadIdx.x;
     const uint64_t base = (uint64_t)blockIdx.x << 12;
     uint64_t gbase = 0;
@@ -1041,28 +1040,50 @@ __global__ __launch_bounds__(256) void k_sums0(const Cx<R>* __restrict__ st, dou
                 hi[b] += (double)a.im;
             }
     }
-    double* out = rec + (uint64_t)blockIdx.x * 26;
-    // total
-    double s = block_sum(tr, lds);
-    if (tid == 0) out[0] = s;
-    s = block_sum(ti, lds);
-    if (tid == 0) out[1] = s;
-    // bits 0..7: threads whose index has the bit
+    // Per wave (butterflies, no barrier): the totals, the sums over lanes
+    // with lane bit q set (q < 6) and the loop-bit sums; one barrier, then
+    // thread 0 adds the four waves in a fixed order.  Bits 6 and 7 of the
+    // amplitude index are the wave index, so their sums are the totals of
+    // waves 1, 3 and of waves 2, 3.
+    constexpr int V = 2 + 12 + 8;
+    double val[V];
+    val[0] = wave_sum(tr);
+    val[1] = wave_sum(ti);
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
+    for (int q = 0; q < 6; q++) {
         const bool on = (tid >> q) & 1u;
-        s = block_sum(on ? tr : 0.0, lds);
-        if (tid == 0) out[2 + 2 * q] = s;
-        s = block_sum(on ? ti : 0.0, lds);
-        if (tid == 0) out[3 + 2 * q] = s;
+        val[2 + 2 * q] = wave_sum(on ? tr : 0.0);
+        val[3 + 2 * q] = wave_sum(on ? ti : 0.0);
     }
-    // bits 8..11: loop index bits
 #pragma unroll
     for (int b = 0; b < 4; b++) {
-        s = block_sum(hr[b], lds);
-        if (tid == 0) out[18 + 2 * b] = s;
-        s = block_sum(hi[b], lds);
-        if (tid == 0) out[19 + 2 * b] = s;
+        val[14 + 2 * b] = wave_sum(hr[b]);
+        val[15 + 2 * b] = wave_sum(hi[b]);
+    }
+    __shared__ double part[4][V];
+    const uint32_t w = tid >> 6;
+    if ((tid & 63u) == 0) {
+#pragma unroll
+        for (int j = 0; j < V; j++) part[w][j] = val[j];
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double* out = rec + (uint64_t)blockIdx.x * 26;
+        auto all = [&](int j) { return (part[0][j] + part[1][j]) + (part[2][j] + part[3][j]); };
+        out[0] = all(0);
+        out[1] = all(1);
+        for (int q = 0; q < 6; q++) {
+            out[2 + 2 * q] = all(2 + 2 * q);
+            out[3 + 2 * q] = all(3 + 2 * q);
+        }
+        out[14] = part[1][0] + part[3][0];  // index bit 6
+        out[15] = part[1][1] + part[3][1];
+        out[16] = part[2][0] + part[3][0];  // index bit 7
+        out[17] = part[2][1] + part[3][1];
+        for (int b = 0; b < 4; b++) {
+            out[18 + 2 * b] = all(14 + 2 * b);
+            out[19 + 2 * b] = all(15 + 2 * b);
+        }
     }
 }
 
