@@ -1117,7 +1117,6 @@ template <typename R, int MB>
 __global__ __launch_bounds__(256) void k_z0(const Cx<R>* __restrict__ st, const uint64_t* __restrict__ masks, int m,
                                             double* __restrict__ part)
 {
-    __shared__ double lds[4];
     const uint32_t tid = threadIdx.x;
     const uint64_t base = (uint64_t)blockIdx.x << 12;
     double acc[MB];
@@ -1139,11 +1138,18 @@ __global__ __launch_bounds__(256) void k_z0(const Cx<R>* __restrict__ st, const 
             acc[j] += f ? -p : p;
         }
     }
-    double* out = part + (uint64_t)blockIdx.x * MB;
+    // wave butterflies, then one barrier and a fixed-order sum of the waves
+    __shared__ double wpart[4][MB];
 #pragma unroll
-    for (int j = 0; j < MB; j++) {
-        const double s = block_sum(acc[j], lds);
-        if (tid == 0) out[j] = s;
+    for (int j = 0; j < MB; j++) acc[j] = wave_sum(acc[j]);
+    if ((tid & 63u) == 0) {
+#pragma unroll
+        for (int j = 0; j < MB; j++) wpart[tid >> 6][j] = acc[j];
+    }
+    __syncthreads();
+    if (tid < (uint32_t)MB) {
+        double* out = part + (uint64_t)blockIdx.x * MB;
+        out[tid] = (wpart[0][tid] + wpart[1][tid]) + (wpart[2][tid] + wpart[3][tid]);
     }
 }
 
