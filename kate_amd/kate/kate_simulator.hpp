@@ -206,10 +206,16 @@ public:
 
     kate_sim* handle() { return sim_; }
 
+    // Kate's callers read the amplitudes through gpu_qbits_states right after
+    // execute(), so the default options run every pass inside execute()
+    // (no deferred trailing permutation passes); a caller that only reads
+    // through this class or the C ABI may pass options without
+    // KATE_PLAN_NO_DEFER.
     static kate_options default_options()
     {
         kate_options o{};
         o.dtype = KATE_C128;
+        o.plan_flags = KATE_PLAN_NO_DEFER;
         return o;
     }
 
