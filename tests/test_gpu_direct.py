@@ -141,3 +141,32 @@ def test_execute_measures_through_trailing_permutations(gpu, dtype):
             ref = O.simulate(n, gates)
             assert np.max(np.abs(sa - ref)) <= 1e-12
             assert np.max(np.abs(pa - O.measure(n, ref))) <= 1e-9
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_runtime_savings_match_the_plain_run(gpu, seed):
+    """Random programs (all gate kinds, some compiled with Kate's swap
+    allocator): execute() with the direct steps, the support skipping and
+    the deferred trailing passes gives the same proba_state and the same
+    state, bit for bit, as with all three switched off."""
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.integers(13, 21))
+    kind = seed % 3
+    if kind == 0:
+        gates = W.random_mixed(n, 150, 200 + seed, max_dense=4) + W.layered(n, 2, seed)
+    elif kind == 1:
+        gates = W.brickwork(n, 6, 300 + seed, longrange_offset=n // 2) + [W.Gate(W.CNOT, (q + 1, q)) for q in range(n - 1)]
+    else:
+        gates = W.layered(n, 4, 400 + seed)
+    say("savings vs plain", seed, n, kind)
+    for alloc in (0, 2):
+        def mk():
+            c = K.Circuit(n, gates)
+            return c.schedule().group(8).allocate(alloc) if alloc else c
+        with K.Simulator(mk(), dtype="c128") as a, \
+                K.Simulator(mk(), dtype="c128", direct=False, support=False, defer=False) as b:
+            pa = a.execute()
+            pb = b.execute()
+            sa, sb = a.state(), b.state()
+        assert np.array_equal(pa, pb), (seed, alloc)
+        assert np.array_equal(sa, sb), (seed, alloc)
