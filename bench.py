@@ -45,34 +45,45 @@ FP64_VALU_PEAK_TFLOPS = 78.6
 
 
 def plan_flops(circuit, dtype, n_int):
-    """VALU flops of the device plan as executed, 2 per FMA-class
-    instruction, per amplitude: 8 * 2^k for a dense k-qubit op (composed
-    1-qubit runs count once; a plain Kronecker op of m one-qubit gates counts
-    m dense 1-qubit gates); 8 per bit plus 8 per diagonal table for a
-    Kronecker op in Euler form; sparse ops (H, CNOT, CRk, Toffoli, phases)
-    count 0."""
+    """VALU flops of each pass of the device plan as executed over every
+    tile, 2 per FMA-class instruction, per amplitude: 8 * 2^k for a dense
+    k-qubit op (composed 1-qubit runs count once; a plain Kronecker op of m
+    one-qubit gates counts m dense 1-qubit gates); 8 per bit plus 8 per
+    diagonal table for a Kronecker op in Euler form; sparse ops (H, CNOT,
+    CRk, Toffoli, phases) count 0.  Returns a list, one entry per pass."""
     import re
     os.environ["KATE_DESCRIBE_OPS"] = "1"
     try:
         text = circuit.plan(dtype)[2]
     finally:
         os.environ.pop("KATE_DESCRIBE_OPS", None)
-    flops = 0.0
+    per_pass = []
     for ln in text.splitlines():
+        if ln.startswith("pass "):
+            per_pass.append(0.0)
+            continue
         if " reg bits=" not in ln and " generic bits=" not in ln and " mfma" not in ln:
             continue
         head = ln.split(" tcols=")[0]
+        f = 0.0
         for kind, code in re.findall(r" (dense|kron|ekron):(\d+)", head):
             m = int(code)
             k = bin(m & 15).count("1")
             if kind == "ekron":
-                flops += 8.0 * (1 << n_int) * (k + 1 + (m >> 4))
+                f += 8.0 * (1 << n_int) * (k + 1 + (m >> 4))
             else:
-                flops += 8.0 * (1 << n_int) * (2 * k if kind == "kron" else (1 << k))
+                f += 8.0 * (1 << n_int) * (2 * k if kind == "kron" else (1 << k))
         if " generic bits=[" in ln or " mfma" in ln:
             k = len(ln.split("bits=[")[1].split("]")[0].split(","))
-            flops += 8.0 * (1 << n_int) * (1 << k)
-    return flops
+            f += 8.0 * (1 << n_int) * (1 << k)
+        per_pass[-1] += f
+    return per_pass
+
+
+def mfma_steps(circuit, dtype, mfma):
+    """Matrix-core micro-steps in the device plan of `circuit`."""
+    text = circuit.plan(dtype, mfma=mfma)[2]
+    return sum(ln.count(" mfma") for ln in text.splitlines())
 
 
 def time_executes(sim, steps, warmup):
@@ -86,22 +97,36 @@ def time_executes(sim, steps, warmup):
     return (time.perf_counter() - t0) * 1e3 / steps
 
 
-def cpu_baseline(n_sample, depth, n_full, gates_full):
-    """The oracle (restated Kate CPU simulator, 14-qubit group staging, one
-    thread per group range) on a bounded sample, extrapolated linearly in
-    gate-amplitude updates to the full circuit."""
+def cpu_baseline(n, depth, budget_s=15.0):
+    """The oracle (plain-C restatement of Kate's CPU simulator: 14-qubit
+    group staging, one thread per group range) on the full-size state of the
+    headline circuit, compiled as Kate's own driver compiles it
+    (gateScheduling, gateGrouping(12), compileDefault(0); main.cpp:30-34).
+    Groups run one at a time from |0...0> until `budget_s` of CPU work; each
+    group is one staged pass over the whole state, so the circuit time is
+    the measured time per group times the group count."""
     from oracle import oracle as O
 
     threads = min(16, len(os.sched_getaffinity(0)))
-    gates = W.layered(n_sample, depth, 1)
-    t0 = time.perf_counter()
-    O.simulate(n_sample, gates, nthreads=threads)
-    sec = time.perf_counter() - t0
-    work_sample = len(gates) * (1 << n_sample)
-    work_full = gates_full * (1 << n_full)
-    return {"value": round(sec * 1e3 * work_full / work_sample, 1), "unit": "ms", "cores": threads, "kind": "port",
-            "sample": f"{n_sample}-qubit depth-{depth} layered circuit ({len(gates)} gates) took {sec:.2f} s; "
-                      f"scaled by gate-amplitude updates to the {n_full}-qubit circuit"}
+    gates = W.layered(n, depth, 1)
+    c = K.Circuit(n, gates).schedule().group(12).compile_default(0)
+    ip, fi = c.permutations()
+    prog = O.Compiled(n, c.gates(), c.groups(), [(1, [])] * len(c.groups()), ip or list(range(n)),
+                      fi or list(range(n)))
+    st = O.init_basis(n)
+    done, sec = 0, 0.0
+    while done < len(prog.groups) and sec < budget_s:
+        t0 = time.perf_counter()
+        O.run_groups(prog, st, done, done + 1, nthreads=threads)
+        sec += time.perf_counter() - t0
+        done += 1
+    del st
+    total = sec * 1e3 * len(prog.groups) / done
+    return {"value": round(total, 1), "unit": "ms", "cores": threads, "kind": "port",
+            "sample": f"{n}-qubit depth-{depth} layered circuit (complex128, {(1 << n) * 16 / 2 ** 30:g} GiB state) compiled as Kate's "
+                      f"driver does (schedule, group(12), compileDefault(0)): {len(prog.groups)} groups; the first "
+                      f"{done} ran in {sec:.2f} s on {threads} threads and the value is that time per group times "
+                      f"{len(prog.groups)}"}
 
 
 def main():
@@ -136,52 +161,67 @@ def main():
     log(f"device: {K.device_info().splitlines()[0]}")
 
     # ---- FP64 plain path (headline) ----
+    # execute() runs every pass of the circuit and leaves the finished state
+    # in HBM, as Kate's execute() does; from |0...0> its passes follow the
+    # state's support (tiles that are still exactly zero are not launched).
     c = K.Circuit(n, gates)
     sim = K.Simulator(c, dtype="c128")
     ms64 = time_executes(sim, args.steps, args.warmup)
     t = sim.timings()
-    # the same with every pass over every tile and every pass run inside
-    # execute() (no support skipping, no deferred trailing permutation
-    # passes): the dense figure, and the one the circuit-level HBM fraction
-    # is quoted on
-    with K.Simulator(K.Circuit(n, gates), dtype="c128", support=False, defer=False) as dense:
-        ms_dense = time_executes(dense, args.steps, args.warmup)
-        t_dense = dense.timings()
-    log(f"fp64 dense (support skipping and deferral off): {ms_dense:.2f} ms/circuit")
-    sim.init_basis(0)
-    pass_ms, remap = sim.run_timed()
+    # the same execute() with an event after every pass: the roofline of
+    # k_pass is quoted on its passes that launch every tile
+    _, pass_ms, tiles = sim.execute_timed()
     sim.synchronize()
     bpp = 2.0 * (1 << max(n, 12)) * 16
-    gate_ms = pass_ms[~remap]
-    mean_pass = float(gate_ms.mean())
+    full = tiles == tiles.max()
+    mean_pass = float(pass_ms[full].mean())
     achieved = bpp / (mean_pass * 1e-3) / 1e9
-    circuit_gbs = bpp * len(pass_ms) / (t_dense["compute_ms"] * 1e-3) / 1e9
-    log(f"fp64: {ms64:.2f} ms/circuit, {len(pass_ms)} passes, mean pass {mean_pass:.3f} ms, "
-        f"{achieved:.0f} GB/s per pass")
-
+    circ_bytes = float(np.sum(bpp * tiles / tiles.max()))
+    circuit_gbs = circ_bytes / (float(pass_ms.sum()) * 1e-3) / 1e9
+    log(f"fp64: {ms64:.2f} ms/circuit, {len(pass_ms)} passes ({int(full.sum())} over every tile), "
+        f"mean full pass {mean_pass:.3f} ms = {achieved:.0f} GB/s")
     flops = plan_flops(c, "c128", max(n, 12))
-    tflops = flops / (float(pass_ms.sum()) * 1e-3) / 1e12
-    log(f"fp64 VALU: {flops / 1e12:.2f} TFLOP planned, {tflops:.1f} TFLOP/s over the passes")
+    full_flops = float(sum(f for f, on in zip(flops, full) if on))
+    tflops = full_flops / (float(pass_ms[full].sum()) * 1e-3) / 1e12
+    log(f"fp64 VALU: {sum(flops) / 1e12:.2f} TFLOP planned, {tflops:.1f} TFLOP/s over the full passes")
+
+    # beside it, not the headline: every tile of every pass (support skipping
+    # off), and execute() leaving its trailing permutation passes for the
+    # next read of the state (deferral on)
+    with K.Simulator(K.Circuit(n, gates), dtype="c128", support=False) as dense:
+        ms_dense = time_executes(dense, args.steps, args.warmup)
+        _, dense_ms, _ = dense.execute_timed()
+    dense_mean = float(dense_ms.mean())
+    log(f"fp64 dense (support skipping off): {ms_dense:.2f} ms/circuit, mean pass {dense_mean:.3f} ms")
+    with K.Simulator(K.Circuit(n, gates), dtype="c128", defer=True) as deferred:
+        ms_deferred = time_executes(deferred, args.steps, args.warmup)
+    log(f"fp64 with the trailing permutation passes deferred: {ms_deferred:.2f} ms/circuit")
 
     out = {
         "metric": METRIC, "value": round(ms64, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms64, 3), "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{n}-qubit depth-{depth} layered circuit (RX, RZ on every qubit, CNOT ladder), "
-                               f"complex128, from |0...0>", "qubits": n, "depth": depth, "gates": len(gates),
-                   "state_gib": (1 << n) * 16 / 2 ** 30},
+                               f"complex128, from |0...0>, every pass run inside execute()", "qubits": n,
+                   "depth": depth, "gates": len(gates), "state_gib": (1 << n) * 16 / 2 ** 30},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": round(bpp * TRAFFIC_RATIO_30Q_LAYERED) if (n, depth) == (30, 20) else None,
                      "kernel": "k_pass<double,12>", "algorithmic_bytes_per_launch": bpp,
-                     "mean_launch_ms": round(mean_pass, 4), "launches": int(len(gate_ms)),
+                     "mean_launch_ms": round(mean_pass, 4), "launches": int(full.sum()),
+                     "run": "the headline execute(): its passes that launch every tile (the others skip "
+                            "tiles that are still zero)",
                      "compute": {"bound": "fp64 valu", "achieved": round(tflops, 2),
                                  "peak": FP64_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
                                  "frac": round(tflops / FP64_VALU_PEAK_TFLOPS, 4),
-                                 "valu_flops": flops}},
-        "passes": int(len(pass_ms)), "init_ms": round(t["init_ms"], 3), "compute_ms": round(t["compute_ms"], 3),
-        "measure_ms": round(t["measure_ms"], 3), "fp64_dense_ms": round(ms_dense, 3),
+                                 "valu_flops": full_flops}},
+        "passes": int(len(pass_ms)), "pass_ms": [round(float(x), 3) for x in pass_ms],
+        "pass_tiles": [int(x) for x in tiles],
+        "init_ms": round(t["init_ms"], 3), "compute_ms": round(t["compute_ms"], 3),
+        "measure_ms": round(t["measure_ms"], 3),
         "circuit_hbm_frac": round(circuit_gbs / HBM_PEAK_GBS, 4),
+        "fp64_dense_ms": round(ms_dense, 3), "fp64_dense_mean_pass_ms": round(dense_mean, 4),
+        "fp64_deferred_ms": round(ms_deferred, 3),
     }
 
     # ---- complex64 + GateMerger(k=5) ----
@@ -214,7 +254,7 @@ def main():
         out["serving"] = serving(args.serve_requests, args.serve_qps, args.serve_qps_high)
         log(f"serving: {out['serving']}")
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(min(n, 22), depth, n, len(gates))
+        out["cpu_baseline"] = cpu_baseline(n, depth)
         log(f"cpu baseline: {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)
 
@@ -237,23 +277,28 @@ def pass_rates(sim, state_bytes):
 
 
 def c2(steps=3):
-    """28-qubit depth-20 layered circuit: FP64 plain beside complex64 +
-    GateMerger(k=5) (VALU and MFMA forms), with max-abs amplitude error and
-    1 - fidelity against the FP64 state."""
+    """28-qubit depth-20 layered circuit: FP64 plain beside complex64
+    (plain, and GateMerger(k=5)-fused on the VALU register path, on FP32
+    matrix-core windows and on split-bf16 matrix-core windows), with
+    max-abs amplitude error and 1 - fidelity against the FP64 state and the
+    number of matrix-core steps each plan holds."""
     n, depth = 28, 20
     gates = W.layered(n, depth, 2)
     res = {"qubits": n, "depth": depth, "gates": len(gates)}
     with K.Simulator(K.Circuit(n, gates), dtype="c128") as ref:
         res["fp64_ms"] = round(time_executes(ref, steps, 1), 3)
         res["fp64_passes"] = ref.timings()["passes"]
-        for label, mfma in (("fp32_fused_valu", 0), ("fp32_fused_mfma", 1), ("fp32_fused_mfma_bf16x3", 3)):
-            c = K.Circuit(n, gates).schedule().fuse(5)
+        legs = (("fp32_plain", False, 0), ("fp32_fused_valu", True, 0), ("fp32_fused_mfma_fp32", True, 2),
+                ("fp32_fused_mfma_bf16x3", True, 3))
+        for label, fuse, mfma in legs:
+            c = K.Circuit(n, gates).schedule().fuse(5) if fuse else K.Circuit(n, gates)
+            steps_mfma = mfma_steps(c, "c64", mfma)
             with K.Simulator(c, dtype="c64", mfma=mfma) as s32:
                 ms = time_executes(s32, steps, 1)
                 _, na, nb, mx = ref.compare(s32)
                 res[label] = {"ms": round(ms, 3), "speedup_vs_fp64": round(res["fp64_ms"] / ms, 3),
                               "max_abs_err": mx, "one_minus_fidelity": 1.0 - ref.fidelity(s32),
-                              "norm_fp32": nb}
+                              "norm_fp32": nb, "mfma_steps": steps_mfma, "passes": s32.timings()["passes"]}
     return res
 
 
@@ -292,32 +337,40 @@ def qulacs(steps=5):
             "kate_mi200_ms": 1035.41, "speedup_vs_kate_mi200": round(1035.41 / ms, 1)}
 
 
-def batched(count):
+def batched(count, check=64):
     """Config C4: `count` Haar brickwork circuits (n log-uniform in [10, 20],
     depth U{8..32}), complex64, one persistent launch per run under a captured
-    graph; fusion width and tile shape from the autotuner on a 256-circuit
-    sample."""
+    graph; fusion width, tile shape and persistent grid from the autotuner on
+    a 256-circuit sample.  <Z_q> of `check` seeded circuits against the FP64
+    oracle, and of every circuit against the same batch run in FP64."""
     from kate_amd.autotune import autotune_batch, build
+    from oracle import oracle as O
 
     t0 = time.perf_counter()
     work = W.batch_workload(count, seed=4)
-    (k, tile, sample_ms), table = autotune_batch(work, dtype="c64", ks=(3, 4, 5, 6), tiles=(11, 12, 13), sample=256)
-    circuits = build(work, k)
-    with K.Batch(circuits, dtype="c64", tile_qubits=tile) as b:
+    best, table = autotune_batch(work, dtype="c64", sample=256)
+    circuits = build(work, best["k"])
+    with K.Batch(circuits, dtype="c64", tile_qubits=best["tile"], low_qubits=best["low"],
+                 wgs_per_cu=best["wgs_per_cu"]) as b:
         b.capture()
         b.run()
         ms = min(b.run() for _ in range(3))
         info = b.info()
         z32, _ = b.expect_z()
-    # <Z_i> of every circuit against the same circuits run unfused in FP64.
     with K.Batch([K.Circuit(n, g) for n, g in work], dtype="c128") as b64:
         b64.run()
         z64, _ = b64.expect_z()
     zerr = max(float(np.abs(np.asarray(a, dtype=np.float64) - np.asarray(r)).max()) for a, r in zip(z32, z64))
+    rng = np.random.default_rng(7)
+    oerr = 0.0
+    for i in rng.choice(len(work), size=min(check, len(work)), replace=False):
+        n, g = work[int(i)]
+        ref = O.expect_z(n, O.simulate(n, g), [1 << q for q in range(n)])
+        oerr = max(oerr, float(np.abs(z32[int(i)] - ref).max()))
     return {"circuits": count, "ms": round(ms, 3), "circuits_per_s": round(count / (ms * 1e-3), 1),
-            "max_z_err_vs_fp64": zerr,
-            "fusion_k": k, "tile_qubits": tile, "tasks": info["tasks"], "persistent_grid": info["grid"],
-            "autotune_table_ms": [[kk, tt, round(mm, 3)] for kk, tt, mm in table],
+            "max_z_err_vs_fp64_batch": zerr, "max_z_err_vs_oracle": oerr, "oracle_checked": min(check, count),
+            "autotuned": {k: v for k, v in best.items() if k != "ms"}, "tasks": info["tasks"],
+            "persistent_grid": info["grid"], "autotune_table": table,
             "setup_s": round(time.perf_counter() - t0, 1)}
 
 

@@ -92,7 +92,7 @@ typedef struct kate_options {
                              dense gate (FP64 composition) for the matrix cores;
                              3 = as 2 with split-bf16 products (x = hi + lo, three
                              products per term) and FP32 accumulation */
-    int32_t fold_remaps;  /* reserved, 0 */
+    int32_t batch_wgs_per_cu; /* kate_batch_*: workgroups per CU of the persistent launch; 0 = 8 */
     int32_t device;       /* HIP device ordinal */
     int32_t micro_bits;   /* amplitudes per thread in register steps = 2^micro_bits:
                              3 (C128 only) or 4; 0 = default per dtype */
@@ -199,6 +199,15 @@ int kate_sim_stream(kate_sim* sim, void** stream);
  * whose execute(true) only prints per-group wall times
  * (HIPcode/simulator.hpp:322-351). */
 int kate_sim_run_timed(kate_sim* sim, float* pass_ms, int32_t* pass_is_remap, int32_t cap, int32_t* npasses);
+/* kate_sim_execute with a HIP event after every pass on the simulator's
+ * stream: pass_ms[i] is pass i's time and pass_tiles[i] the number of tiles
+ * it launched (fewer than 2^(n - tile qubits) when the run from |0...0>
+ * skips tiles that are known to be zero); either may be NULL.  The passes
+ * run are exactly those of kate_sim_execute (*npasses of them).  No
+ * counterpart in Kate, whose execute(true) only prints per-group wall times
+ * (HIPcode/simulator.hpp:322-351). */
+int kate_sim_execute_timed(kate_sim* sim, const double* theta_phi_in, double* theta_phi_out, float* pass_ms,
+                           uint64_t* pass_tiles, int32_t cap, int32_t* npasses);
 /* Pass plan as text (one line per pass), for inspection and tests. */
 int kate_sim_describe(kate_sim* sim, char* buf, size_t len);
 /* The same plan without a device (no counterpart in Kate, whose grouping is
@@ -264,8 +273,9 @@ int kate_batch_capture(kate_batch* batch);
 int kate_batch_run(kate_batch* batch);
 int kate_batch_info(const kate_batch* batch, int32_t* count, int64_t* z_len, int32_t* tasks, int32_t* passes,
                     int32_t* grid);
-/* <Z_q> of every circuit, concatenated in circuit order (z_len values);
- * norms (may be NULL) receives each circuit's sum |a|^2. */
+/* <Z_q> of every circuit for real qubit q (the circuit's
+ * final_inverse_permutation applied), concatenated in circuit order (z_len
+ * values); norms (may be NULL) receives each circuit's sum |a|^2. */
 int kate_batch_expect_z(kate_batch* batch, double* out, int64_t len, double* norms);
 int kate_batch_last_ms(const kate_batch* batch, float* ms);
 

@@ -36,7 +36,7 @@ class _GateRec(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [("dtype", C.c_int32), ("tile_qubits", C.c_int32), ("low_qubits", C.c_int32),
-                ("use_mfma", C.c_int32), ("fold_remaps", C.c_int32), ("device", C.c_int32),
+                ("use_mfma", C.c_int32), ("batch_wgs_per_cu", C.c_int32), ("device", C.c_int32),
                 ("micro_bits", C.c_int32), ("plan_flags", C.c_int32)]
 
 
@@ -72,7 +72,7 @@ EXPORTS = [
     "kate_sim_measure_proba", "kate_sim_execute", "kate_sim_expect_z", "kate_sim_norm", "kate_sim_renormalize",
     "kate_sim_read_state", "kate_sim_write_state", "kate_sim_device_state", "kate_sim_last_timings",
     "kate_sim_synchronize", "kate_sim_stream", "kate_sim_describe", "kate_plan_describe", "kate_sim_run_timed",
-    "kate_sim_overlap",
+    "kate_sim_overlap", "kate_sim_execute_timed",
     "kate_circuit_create", "kate_circuit_destroy", "kate_circuit_append", "kate_circuit_schedule",
     "kate_circuit_fuse", "kate_circuit_group", "kate_circuit_allocate", "kate_circuit_compile_default",
     "kate_circuit_compile_opt", "kate_circuit_export_qcx", "kate_circuit_info", "kate_circuit_gate",
@@ -136,6 +136,7 @@ def lib():
         L.kate_sim_create_from_circuit.argtypes = [VP, C.POINTER(Options), C.POINTER(VP)]
         L.kate_sim_overlap.argtypes = [VP, VP, DP]
         L.kate_sim_run_timed.argtypes = [VP, C.POINTER(C.c_float), I32P, C.c_int32, I32P]
+        L.kate_sim_execute_timed.argtypes = [VP, DP, DP, C.POINTER(C.c_float), U64P, C.c_int32, I32P]
         L.kate_plan_describe.argtypes = [VP, C.POINTER(Options), I32P, I32P, C.c_char_p, C.c_size_t]
         L.kate_circuit_plan.argtypes = [VP, C.POINTER(Options), I32P, I32P, C.c_char_p, C.c_size_t]
         L.kate_batch_create.argtypes = [VP, C.c_int32, C.POINTER(Options), C.POINTER(VP)]
@@ -305,7 +306,12 @@ class Simulator:
 
     def __init__(self, circuit: Circuit, dtype: str = "c128", tile_qubits: int = 0, low_qubits: int = 0,
                  device: int = 0, mfma: int = 0, micro_bits: int = 0, euler: bool = True, direct: bool = True,
-                 support: bool = True, defer: bool = True):
+                 support: bool = True, defer: bool = False):
+        """`defer=True` lets execute() leave the trailing passes that only
+        permute amplitudes (CNOT layers at the end of the circuit) unrun:
+        its proba_state reads through them and any later read of the state
+        runs them first.  Off by default, as Kate's execute() leaves the
+        finished state in memory."""
         self.n = circuit.n
         self.dtype = {"c128": KATE_C128, "c64": KATE_C64}[dtype]
         self.amp = np.complex128 if self.dtype == KATE_C128 else np.complex64
@@ -356,6 +362,20 @@ class Simulator:
         _check(lib().kate_sim_run_timed(self._h, ms.ctypes.data_as(C.POINTER(C.c_float)), _p(rm, I32P), cap,
                                         C.byref(n)), "run_timed")
         return ms[:n.value].astype(np.float64), rm[:n.value].astype(bool)
+
+    def execute_timed(self, proba_in: Optional[np.ndarray] = None) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """execute() with a HIP event after every pass: (proba_state, ms per
+        pass, tiles launched per pass) for exactly the passes execute() runs."""
+        out = np.zeros(2 * self.n)
+        pin = None if proba_in is None else np.ascontiguousarray(proba_in, dtype=np.float64).ravel()
+        cap = 1 << 16
+        ms = np.zeros(cap, dtype=np.float32)
+        tiles = np.zeros(cap, dtype=np.uint64)
+        n = C.c_int32()
+        _check(lib().kate_sim_execute_timed(self._h, _p(pin, DP) if pin is not None else None, _p(out, DP),
+                                            ms.ctypes.data_as(C.POINTER(C.c_float)), _p(tiles, U64P), cap,
+                                            C.byref(n)), "execute_timed")
+        return out.reshape(self.n, 2), ms[:n.value].astype(np.float64), tiles[:n.value].copy()
 
     def compare(self, other: "Simulator") -> Tuple[complex, float, float, float]:
         """(<self|other>, <self|self>, <other|other>, max |self_x - other_x|),
@@ -429,12 +449,13 @@ class Batch:
     starts at |0...0>; a run returns <Z_q> for every qubit of every circuit."""
 
     def __init__(self, circuits: Sequence[Circuit], dtype: str = "c64", tile_qubits: int = 0, low_qubits: int = 0,
-                 device: int = 0, mfma: int = 0, micro_bits: int = 0, euler: bool = True, direct: bool = True):
+                 device: int = 0, mfma: int = 0, micro_bits: int = 0, euler: bool = True, direct: bool = True,
+                 wgs_per_cu: int = 0):
         self.circuits = list(circuits)
         self.nq = [c.n for c in self.circuits]
         arr = (VP * len(self.circuits))(*[c._h for c in self.circuits])
-        opt = Options({"c128": KATE_C128, "c64": KATE_C64}[dtype], tile_qubits, low_qubits, mfma, 0, device,
-                      micro_bits, _plan_flags(euler, direct))
+        opt = Options({"c128": KATE_C128, "c64": KATE_C64}[dtype], tile_qubits, low_qubits, mfma, wgs_per_cu,
+                      device, micro_bits, _plan_flags(euler, direct))
         h = VP()
         _check(lib().kate_batch_create_from_circuits(arr, len(self.circuits), C.byref(opt), C.byref(h)),
                "kate_batch_create")

@@ -51,6 +51,9 @@ struct kate_batch {
     uint32_t count = 0;
     std::vector<int32_t> nq, nq_int;
     std::vector<int64_t> zoff;
+    // final_inverse_permutation of every circuit, concatenated at zoff: the
+    // kernel sums <Z> per virtual position p, which is real qubit fip[p]
+    std::vector<int32_t> fip;
     int64_t zlen = 0;
     uint64_t total_amps = 0;
     uint32_t ntasks = 0, npasses = 0;
@@ -192,6 +195,7 @@ int kate_batch_create(const kate_program* programs, int32_t count, const kate_op
         b->nq_int.push_back(plan.nq_int);
         b->zoff.push_back(b->zlen);
         b->zlen += prog.nqubits;
+        b->fip.insert(b->fip.end(), prog.final_inverse_permutation.begin(), prog.final_inverse_permutation.end());
     }
     b->npasses = (uint32_t)passes.size();
     // tasks by pass level, then circuit, then tile.  Every state starts at
@@ -257,7 +261,8 @@ int kate_batch_create(const kate_program* programs, int32_t count, const kate_op
         hipDeviceProp_t prop;
         step(hipGetDeviceProperties(&prop, b->device), "hipGetDeviceProperties");
         // persistent grid: enough workgroups to fill every CU several times over
-        b->grid = (unsigned)std::min<uint64_t>(b->ntasks, (uint64_t)prop.multiProcessorCount * 8);
+        const uint64_t per_cu = opt.batch_wgs_per_cu > 0 ? (uint64_t)opt.batch_wgs_per_cu : 8;
+        b->grid = (unsigned)std::min<uint64_t>(b->ntasks, (uint64_t)prop.multiProcessorCount * per_cu);
         if (b->grid == 0) b->grid = 1;
     }
     if (rc != KATE_OK) {
@@ -328,7 +333,14 @@ int kate_batch_expect_z(kate_batch* b, double* out, int64_t len, double* norms)
 {
     if (!b || !out || len != b->zlen) return bfail(KATE_ERR_ARG, "output length must equal the total qubit count");
     KB_HIP(hipSetDevice(b->device));
-    KB_HIP(hipMemcpy(out, b->d_z, (size_t)len * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<double> byvirt((size_t)len);
+    KB_HIP(hipMemcpy(byvirt.data(), b->d_z, (size_t)len * sizeof(double), hipMemcpyDeviceToHost));
+    // virtual position p of circuit c holds real qubit fip[zoff[c] + p]
+    for (uint32_t c = 0; c < b->count; c++)
+        for (int32_t p = 0; p < b->nq[c]; p++) {
+            const size_t at = (size_t)b->zoff[c] + (size_t)p;
+            out[(size_t)b->zoff[c] + (size_t)b->fip[at]] = byvirt[at];
+        }
     if (norms) KB_HIP(hipMemcpy(norms, b->d_norms, b->count * sizeof(double), hipMemcpyDeviceToHost));
     return KATE_OK;
 }

@@ -212,7 +212,10 @@ void release(kate_sim* s)
 // the first pass makes its tile in registers and, unless the support
 // skipping is off, the passes follow the support of the state).
 // Passes [begin, end) -- all of them by default.
-int run_passes(kate_sim* s, bool from_zero = false, size_t begin = 0, size_t end = (size_t)-1)
+// With `after` (events, one per pass in [begin, end)) an event is recorded
+// after each pass and `tiles` receives each pass's launched tile count.
+int run_passes(kate_sim* s, bool from_zero = false, size_t begin = 0, size_t end = (size_t)-1,
+               hipEvent_t* after = nullptr, uint64_t* tiles = nullptr)
 {
     kate::DevPlan dp{s->d_micro, s->d_ops, s->d_coef, s->d_raw};
     end = std::min(end, s->plan.passes.size());
@@ -220,6 +223,8 @@ int run_passes(kate_sim* s, bool from_zero = false, size_t begin = 0, size_t end
         const kate::PassArgs pa = from_zero ? kate::zero_start_pass(s->plan, i, s->support, end) : s->plan.passes[i];
         int e = kate::launch_pass(s->dtype, s->plan.tile_bits, s->d_state, s->nq_int, pa, dp, s->stream);
         if (e) return fail(KATE_ERR_HIP, std::string("pass launch: ") + hipGetErrorString((hipError_t)e));
+        if (tiles) tiles[i - begin] = 1ull << pa.nrest;
+        if (after) KATE_HIP(hipEventRecord(after[i - begin], s->stream));
     }
     return KATE_OK;
 }
@@ -354,6 +359,7 @@ std::string plan_options(const kate_options& opt, kate::PlanOptions& po)
     if (po.micro_bits == 3 && opt.use_mfma) return "micro_bits 3 excludes use_mfma";
     if (po.tile_bits < 10 || po.tile_bits > 13) return "tile_qubits must be in 10..13";
     if (po.low_qubits < 0 || po.low_qubits > po.tile_bits) return "low_qubits out of range";
+    if (opt.batch_wgs_per_cu < 0 || opt.batch_wgs_per_cu > 32) return "batch_wgs_per_cu must be in 0..32";
     return "";
 }
 }  // namespace kate
@@ -594,10 +600,17 @@ int kate_sim_measure_proba(kate_sim* s, double* tp)
     return measure(s, tp);  // reads through pending trailing passes
 }
 
-int kate_sim_execute(kate_sim* s, const double* tp_in, double* tp_out)
+namespace {
+
+// kate_sim_execute, optionally with an event after every pass (pass_ev,
+// one per pass the run launches: execute_pass_count of them).
+size_t execute_pass_count(const kate_sim* s)
 {
-    if (!s) return fail(KATE_ERR_ARG, "null simulator");
-    KATE_HIP(hipSetDevice(s->device));
+    return (s->defer && s->plan.trail_begin > 0) ? s->plan.trail_begin : s->plan.passes.size();
+}
+
+int execute_impl(kate_sim* s, const double* tp_in, double* tp_out, hipEvent_t* pass_ev, uint64_t* pass_tiles)
+{
     KATE_HIP(hipEventRecord(s->ev[0], s->stream));
     // |0...0> is made by the first pass itself (no init pass over HBM) unless
     // there is no pass to make it
@@ -610,8 +623,8 @@ int kate_sim_execute(kate_sim* s, const double* tp_in, double* tp_out)
     KATE_HIP(hipEventRecord(s->ev[1], s->stream));
     // trailing permutation passes are left for whoever reads the state next;
     // the measurement below reads through them
-    const size_t n_run = (s->defer && s->plan.trail_begin > 0) ? s->plan.trail_begin : s->plan.passes.size();
-    rc = run_passes(s, fused_init, 0, n_run);
+    const size_t n_run = execute_pass_count(s);
+    rc = run_passes(s, fused_init, 0, n_run, pass_ev, pass_tiles);
     if (rc) return rc;
     s->pending = n_run < s->plan.passes.size();
     KATE_HIP(hipEventRecord(s->ev[2], s->stream));
@@ -628,6 +641,43 @@ int kate_sim_execute(kate_sim* s, const double* tp_in, double* tp_out)
     s->last.compute_ms = b;
     s->last.measure_ms = c;
     return KATE_OK;
+}
+
+}  // namespace
+
+int kate_sim_execute(kate_sim* s, const double* tp_in, double* tp_out)
+{
+    if (!s) return fail(KATE_ERR_ARG, "null simulator");
+    KATE_HIP(hipSetDevice(s->device));
+    return execute_impl(s, tp_in, tp_out, nullptr, nullptr);
+}
+
+int kate_sim_execute_timed(kate_sim* s, const double* tp_in, double* tp_out, float* pass_ms, uint64_t* pass_tiles,
+                           int32_t cap, int32_t* npasses)
+{
+    if (!s || cap < 0) return fail(KATE_ERR_ARG, "bad arguments");
+    KATE_HIP(hipSetDevice(s->device));
+    const size_t np = execute_pass_count(s);
+    if (npasses) *npasses = (int32_t)np;
+    std::vector<hipEvent_t> ev(np, nullptr);
+    std::vector<uint64_t> tiles(np, 0);
+    int rc = KATE_OK;
+    for (auto& e : ev)
+        if (rc == KATE_OK && hipEventCreate(&e) != hipSuccess) rc = fail(KATE_ERR_HIP, "hipEventCreate");
+    if (rc == KATE_OK) rc = execute_impl(s, tp_in, tp_out, ev.data(), tiles.data());
+    // pass i runs between the end of the init (ev[1]) or of pass i-1 and ev[i]
+    for (size_t i = 0; i < np && rc == KATE_OK; i++) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, i ? ev[i - 1] : s->ev[1], ev[i]) != hipSuccess)
+            rc = fail(KATE_ERR_HIP, "hipEventElapsedTime");
+        if ((int32_t)i < cap) {
+            if (pass_ms) pass_ms[i] = ms;
+            if (pass_tiles) pass_tiles[i] = tiles[i];
+        }
+    }
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    return rc;
 }
 
 int kate_sim_expect_z(kate_sim* s, const uint64_t* masks, int32_t count, double* out)

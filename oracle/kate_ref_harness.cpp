@@ -11,8 +11,10 @@
 #include "Kate.hpp"
 #include "BenchCircuits.hpp"
 
+#include <chrono>
 #include <cstdint>
 #include <cstring>
+#include <iostream>
 
 namespace {
 struct Handle {
@@ -124,6 +126,25 @@ int kref_simulate(void* h, const double* proba_in, double* state_out, double* pr
         }
     return 0;
 }
+
+// CPUSimulator::execute(true) on a copy of the circuit: Kate prints its own
+// "Initialization / Computation / measurement time" lines
+// (CPUcode/simulator.hpp:86-97); *wall_ms is the wall time of the execute
+// call (the simulator's construction and state allocation excluded).
+int kref_time_execute(void* h, double* wall_ms)
+{
+    Kate::QuantumCircuit qc = H(h)->c;
+    Kate::CPUSimulator sim(qc, 0);
+    const auto t0 = std::chrono::steady_clock::now();
+    Kate::proba_state res = sim.execute(true);
+    const auto t1 = std::chrono::steady_clock::now();
+    std::cout.flush();
+    if (wall_ms) *wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    return res.val.empty() ? -1 : 0;
+}
+
+// Circuit::exportqcx (backendagnosticcode/Circuit.hpp:763-774).
+void kref_export_qcx(void* h, const char* path) { H(h)->c.exportqcx(path); }
 
 // CPUmergeGate on gates [first, last) of the circuit.  Returns k.
 int kref_merge(void* h, int first, int last, int* out_qubits, double* out_mat)

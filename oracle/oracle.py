@@ -143,6 +143,38 @@ def run(prog: Compiled, proba_in: Optional[np.ndarray] = None, nthreads: int = 0
     return st
 
 
+def run_groups(prog: Compiled, state: np.ndarray, first: int, last: int, nthreads: int = 0):
+    """Runs compute groups [first, last) of a swap-free program on `state`
+    in place (the slice of CPUSimulator::execute's instruction loop that
+    computes those groups)."""
+    if any(k != 1 for k, _ in prog.instructions):
+        raise ValueError("run_groups takes swap-free programs")
+    g0 = prog.groups[first - 1][0] if first > 0 else 0
+    g1 = prog.groups[last - 1][0]
+    sub = Compiled(prog.nqubits, list(prog.gates[g0:g1]),
+                   [(e - g0, m) for e, m in prog.groups[first:last]], [(1, [])] * (last - first),
+                   prog.initial_permutation, prog.final_inverse_permutation)
+    fg = FlatGates(sub.gates)
+    gend = np.array([e for e, _ in sub.groups], dtype=np.int32)
+    gmask = np.array([m for _, m in sub.groups], dtype=np.uint64)
+    kinds = np.ones(len(sub.groups), dtype=np.int32)
+    zeros = np.zeros(len(sub.groups), dtype=np.int32)
+    pairs = np.zeros(1, dtype=np.int32)
+    p = _Program(sub.nqubits, fg.struct(), len(sub.groups), _p(gend, I32P), _p(gmask, U64P), len(sub.groups),
+                 _p(kinds, I32P), _p(zeros, I32P), _p(zeros, I32P), _p(pairs, I32P))
+    nthreads = nthreads or min(16, len(os.sched_getaffinity(0)))
+    rc = lib().orc_run(C.byref(p), _p(state.view(np.float64), DP), nthreads)
+    if rc != 0:
+        raise RuntimeError(f"oracle run failed ({rc})")
+
+
+def init_basis(nqubits: int) -> np.ndarray:
+    """|0...0> (CPUcode/simulator.hpp initialize)."""
+    st = np.zeros(1 << nqubits, dtype=np.complex128)
+    lib().orc_init_basis(nqubits, _p(st.view(np.float64), DP))
+    return st
+
+
 def simulate(nqubits: int, gates: Sequence, proba_in: Optional[np.ndarray] = None, nthreads: int = 0) -> np.ndarray:
     return run(Compiled.naive(nqubits, gates), proba_in, nthreads)
 
@@ -214,6 +246,8 @@ def ref():
         r.kref_perms.argtypes = [C.c_void_p, I32P, I32P]
         r.kref_simulate.argtypes = [C.c_void_p, DP, DP, DP]
         r.kref_merge.argtypes = [C.c_void_p, C.c_int, C.c_int, I32P, DP]
+        r.kref_time_execute.argtypes = [C.c_void_p, DP]
+        r.kref_export_qcx.argtypes = [C.c_void_p, C.c_char_p]
         r.kref_qft.restype = C.c_void_p
         r.kref_qft.argtypes = [C.c_int]
         r.kref_qulacs.restype = C.c_void_p
@@ -323,6 +357,18 @@ class KateRef:
         self.r.kref_simulate(self.h, _p(pin, DP) if pin is not None else None, _p(st.view(np.float64), DP),
                              _p(pr, DP))
         return st, pr.reshape(self.n, 2)
+
+    def time_execute(self) -> float:
+        """Wall ms of Kate's CPUSimulator::execute(true) (which prints its own
+        init / computation / measurement split on stdout)."""
+        ms = np.zeros(1)
+        if self.r.kref_time_execute(self.h, _p(ms, DP)) != 0:
+            raise RuntimeError("Kate's CPU simulator returned no proba_state")
+        return float(ms[0])
+
+    def export_qcx(self, path: str):
+        """Kate's Circuit::exportqcx."""
+        self.r.kref_export_qcx(self.h, path.encode())
 
     def merge(self, first, last):
         qs = np.zeros(64, dtype=np.int32)

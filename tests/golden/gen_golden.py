@@ -66,8 +66,13 @@ def write(name, meta, arrays):
     np.savez_compressed(os.path.join(OUT, name + ".npz"), **arrays)
 
 
-def main():
-    circuits = {
+QCX_CASES = [("mixed8", "raw"), ("mixed8", "sched_fuse5"), ("qft9", "raw"), ("qft9", "sched_group5_alloc2"),
+             ("mixed11_dense4", "raw"), ("mixed11_dense4", "sched_fuse3_group6"), ("qulacs7", "alloc0"),
+             ("brick12", "sched_group5_alloc2")]
+
+
+def circuits():
+    return {
         "mixed8": (8, W.random_mixed(8, 80, 3)),
         "layered10": (10, W.layered(10, 4, 5)),
         "qft9": (9, W.qft(9)),
@@ -75,8 +80,35 @@ def main():
         "qulacs7": (7, W.qulacs_bench(7)),
         "mixed11_dense4": (11, W.random_mixed(11, 60, 7, max_dense=4)),
     }
-    written = 0
-    for cname, (n, gates) in circuits.items():
+
+
+def write_qcx():
+    """Kate's Circuit::exportqcx (Circuit.hpp:763-774) of circuits after
+    the named passes: kate_qcx_<circuit>_<pipeline>.qcx, byte for byte."""
+    cs = circuits()
+    for cname, pname in QCX_CASES:
+        n, gates = cs[cname]
+        ref = O.KateRef(n, gates)
+        apply(ref, PIPELINES[pname])
+        ref.export_qcx(os.path.join(OUT, f"kate_qcx_{cname}_{pname}.qcx"))
+    return len(QCX_CASES)
+
+
+def main():
+    if "--qcx-only" in sys.argv:
+        print(f"wrote {write_qcx()} QCX fixtures to {OUT}")
+        return
+    circuits_ = {
+        "mixed8": (8, W.random_mixed(8, 80, 3)),
+        "layered10": (10, W.layered(10, 4, 5)),
+        "qft9": (9, W.qft(9)),
+        "brick12": (12, W.brickwork(12, 8, 2, longrange_every=4, longrange_offset=5)),
+        "qulacs7": (7, W.qulacs_bench(7)),
+        "mixed11_dense4": (11, W.random_mixed(11, 60, 7, max_dense=4)),
+    }
+    assert circuits_.keys() == circuits().keys()
+    written = write_qcx()
+    for cname, (n, gates) in circuits_.items():
         for pname, steps in PIPELINES.items():
             ref = O.KateRef(n, gates)
             apply(ref, steps)

@@ -80,3 +80,50 @@ def test_batch_result_independent_of_company(gpu):
         z, _ = b.expect_z()
     for a, m in zip(alone, z[:3]):
         assert np.array_equal(a, m)
+
+
+def test_batch_compiled_circuits_report_real_qubits(gpu):
+    """Circuits compiled by Kate's passes (scheduling, grouping and the
+    allocate() swap plan, which relabels qubits) give <Z_q> per real qubit q,
+    as the oracle computes it on the uncompiled circuit."""
+    say("batch compiled")
+    work = W.batch_workload(12, seed=21, nmin=12, nmax=16, dmin=6, dmax=12)
+    work.append((14, W.layered(14, 3, 5)))
+    circuits = [K.Circuit(n, g).schedule().group(8).allocate(2) for n, g in work]
+    assert any(c.permutations()[1] != list(range(c.n)) for c in circuits)
+    for dtype, tol in (("c128", 1e-12), ("c64", 2e-6)):
+        with K.Batch(circuits, dtype=dtype) as b:
+            b.run()
+            z, _ = b.expect_z()
+        for (n, gates), zc in zip(work, z):
+            err = np.max(np.abs(zc - reference_z(n, gates)))
+            assert err < tol, (dtype, n, err)
+
+
+def test_full_c4_batch_with_autotuned_shape(gpu):
+    """The full batched workload (4096 Haar brickwork circuits, n
+    log-uniform in [10, 20], depth 8..32, complex64) in one persistent
+    launch, in the shape the autotuner picks on a sample: <Z_q> of 64 seeded
+    circuits against the FP64 oracle, and every norm."""
+    from kate_amd.autotune import autotune_batch, build
+
+    say("full C4: autotune")
+    work = W.batch_workload(4096, seed=4)
+    best, table = autotune_batch(work, dtype="c64", sample=64, reps=1)
+    say("full C4: picked", best)
+    with K.Batch(build(work, best["k"]), dtype="c64", tile_qubits=best["tile"], low_qubits=best["low"],
+                 wgs_per_cu=best["wgs_per_cu"]) as b:
+        b.capture()
+        b.run()
+        z, norms = b.expect_z()
+    assert np.max(np.abs(norms - 1.0)) < 1e-5
+    rng = np.random.default_rng(7)
+    idx = rng.choice(len(work), size=64, replace=False)
+    worst = 0.0
+    for j, i in enumerate(idx):
+        n, gates = work[int(i)]
+        err = float(np.max(np.abs(z[int(i)] - reference_z(n, gates))))
+        worst = max(worst, err)
+        if j % 16 == 15:
+            say(f"full C4: {j + 1} of 64 circuits checked, worst <Z> error {worst:.2e}")
+        assert err < 2e-6, (int(i), n, err)

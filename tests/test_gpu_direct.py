@@ -127,7 +127,7 @@ def test_execute_measures_through_trailing_permutations(gpu, dtype):
                      (24, W.haar_layers(24, 1, 43) + W.layered(24, 2, 44))):
         _, _, text = K.Circuit(n, gates).plan(dtype)
         assert " micro=0 " in text.splitlines()[-1] + " "  # the last pass only permutes
-        with K.Simulator(K.Circuit(n, gates), dtype=dtype) as a, \
+        with K.Simulator(K.Circuit(n, gates), dtype=dtype, defer=True) as a, \
                 K.Simulator(K.Circuit(n, gates), dtype=dtype, defer=False) as b:
             pa = a.execute()
             pb = b.execute()
@@ -163,7 +163,7 @@ def test_runtime_savings_match_the_plain_run(gpu, seed):
         def mk():
             c = K.Circuit(n, gates)
             return c.schedule().group(8).allocate(alloc) if alloc else c
-        with K.Simulator(mk(), dtype="c128") as a, \
+        with K.Simulator(mk(), dtype="c128", defer=True) as a, \
                 K.Simulator(mk(), dtype="c128", direct=False, support=False, defer=False) as b:
             pa = a.execute()
             pb = b.execute()

@@ -207,3 +207,21 @@ def test_plan_flags_validation_and_trailing_passes():
     assert rc != 0 and "plan_flags" in K.lib().kate_last_error().decode()
     lines = [ln for ln in circ.plan("c128")[2].splitlines() if ln.startswith("pass")]
     assert " micro=0 " in lines[-1] + " "
+
+
+QCX_FIXTURES = fixtures("kate_qcx_*.qcx")
+
+
+@pytest.mark.parametrize("path", QCX_FIXTURES, ids=lambda p: p.split("/")[-1][9:-4])
+def test_qcx_export_equals_kate(path, tmp_path):
+    """Circuit::exportqcx (Circuit.hpp:763-774, Gate::transformqcx
+    basic_host_types.hpp:270-295) after Kate's passes, byte for byte against
+    the file Kate's own build wrote for the same input gates and passes."""
+    assert QCX_FIXTURES
+    name = path.split("/")[-1][len("kate_qcx_"):-len(".qcx")]
+    z, meta = load(path[: -len(path.split("/")[-1])] + f"kate_{name}.npz")
+    c = run_pipeline(K.Circuit(meta["n"], unpack_gates(z, "in_")), meta["pipeline"])
+    out = tmp_path / "ours.qcx"
+    c.export_qcx(str(out))
+    with open(path, "rb") as f:
+        assert out.read_bytes() == f.read()
