@@ -147,6 +147,10 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-qps", type=float, default=4.0)
     ap.add_argument("--serve-qps-high", type=float, default=1000.0)
+    ap.add_argument("--passes-only", choices=("execute", "dense"), default=None,
+                    help="for profilers: one execute() of the headline circuit (execute), or one run of every "
+                         "pass over every tile (dense), after one untimed warm-up execute(); prints the pass times")
+    ap.add_argument("--dtype", choices=("c128", "c64"), default="c128", help="state type of --passes-only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the simulate path it measures is Kate's single-GPU call "
@@ -159,6 +163,19 @@ def main():
     n, depth = args.qubits, args.depth
     gates = W.layered(n, depth, 1)
     log(f"device: {K.device_info().splitlines()[0]}")
+    if args.passes_only:
+        with K.Simulator(K.Circuit(n, gates), dtype=args.dtype, support=args.passes_only == "execute") as sim:
+            sim.execute()
+            if args.passes_only == "execute":
+                _, ms, tiles = sim.execute_timed()
+            else:
+                sim.init_basis(0)
+                ms, _ = sim.run_timed()
+            sim.synchronize()
+        print(json.dumps({"mode": args.passes_only, "dtype": args.dtype, "qubits": n, "depth": depth,
+                          "passes": len(ms), "sum_ms": round(float(ms.sum()), 3),
+                          "pass_ms": [round(float(x), 3) for x in ms]}), flush=True)
+        return
 
     # ---- FP64 plain path (headline) ----
     # execute() runs every pass of the circuit and leaves the finished state

@@ -116,7 +116,13 @@ enum {
     /* execute(): run the trailing passes that only permute amplitudes
        (CNOT layers at the end of a circuit) instead of measuring through
        them and running them when the state is next read */
-    KATE_PLAN_NO_DEFER = 8
+    KATE_PLAN_NO_DEFER = 8,
+    /* apply dense 2-qubit gates that factor as (CNOT x) (A (x) B) (x CNOT)
+       (what Kate's gateFusion makes of two neighbouring 1-qubit gates and
+       the CNOT between them) as dense 4x4 products instead of as their
+       factors (1-qubit gates composed with their neighbours, the CNOT a
+       layout update; same result to rounding) */
+    KATE_PLAN_NO_SPLIT = 16
 };
 
 /* Wall-clock split of the last execute, as Kate's execute(true) prints it

@@ -165,6 +165,138 @@ std::vector<double> permute_matrix(const double* m, int k, const int* order)
     return out;
 }
 
+// Splits dense 2-qubit gates that are a Kronecker product of two 1-qubit
+// gates, optionally times one CNOT on either side (P * (A x B) or
+// (A x B) * P, P a CNOT between the two qubits or the identity), into those
+// factors.  Kate's gateFusion produces such gates when it merges the 1-qubit
+// gates of two neighbouring qubits with the CNOT between them
+// (backendagnosticcode/Circuit.hpp:99-175); as one dense 4x4 they need both
+// qubits in the same register step and tie every gate before and after them
+// on either qubit together, while the factors are composed with their
+// neighbours on each qubit and the CNOT costs nothing (it becomes a layout
+// update).  A gate is split only when the factors reproduce it to 1e-15
+// max-abs (FP64 composition rounding), so results change by rounding only.
+bool kron_factors(const double* m, double (&a)[8], double (&b)[8])
+{
+    // R[(r1,c1)][(r0,c0)] = M[r1*2+r0][c1*2+c0] (index bit 0 = qubits[0])
+    double R[4][4][2];
+    for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++) {
+            const int i = ((r >> 1) << 1) | (c >> 1), j = ((r & 1) << 1) | (c & 1);
+            R[i][j][0] = m[2 * (r * 4 + c)];
+            R[i][j][1] = m[2 * (r * 4 + c) + 1];
+        }
+    int bi = 0, bj = 0;
+    double best = -1;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            const double v = R[i][j][0] * R[i][j][0] + R[i][j][1] * R[i][j][1];
+            if (v > best) {
+                best = v;
+                bi = i;
+                bj = j;
+            }
+        }
+    if (best <= 0) return false;
+    // a = column bj of R, b = row bi of R / R[bi][bj]
+    const double pr = R[bi][bj][0], pi = R[bi][bj][1], pn = pr * pr + pi * pi;
+    double av[4][2], bv[4][2];
+    for (int i = 0; i < 4; i++) {
+        av[i][0] = R[i][bj][0];
+        av[i][1] = R[i][bj][1];
+        bv[i][0] = (R[bi][i][0] * pr + R[bi][i][1] * pi) / pn;
+        bv[i][1] = (R[bi][i][1] * pr - R[bi][i][0] * pi) / pn;
+    }
+    double na = 0, nb = 0;
+    for (int i = 0; i < 4; i++) {
+        na += av[i][0] * av[i][0] + av[i][1] * av[i][1];
+        nb += bv[i][0] * bv[i][0] + bv[i][1] * bv[i][1];
+    }
+    const double t = std::sqrt(std::sqrt(nb / na));  // balance the two factors
+    for (int i = 0; i < 4; i++) {
+        a[2 * i] = av[i][0] * t;
+        a[2 * i + 1] = av[i][1] * t;
+        b[2 * i] = bv[i][0] / t;
+        b[2 * i + 1] = bv[i][1] / t;
+    }
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            const double re = a[2 * i] * b[2 * j] - a[2 * i + 1] * b[2 * j + 1];
+            const double im = a[2 * i] * b[2 * j + 1] + a[2 * i + 1] * b[2 * j];
+            if (std::fabs(re - R[i][j][0]) > 2e-15 || std::fabs(im - R[i][j][1]) > 2e-15) return false;
+        }
+    return true;
+}
+
+// The 4x4 permutation of a CNOT between the two qubits of a 2-qubit gate
+// (matrix index bit i = qubits[i]); ctl = the control's index bit.
+int cnot_perm(int ctl, int r)
+{
+    const int tgt = 1 - ctl;
+    return ((r >> ctl) & 1) ? r ^ (1 << tgt) : r;
+}
+
+// Program with every splittable dense 2-qubit gate replaced by its factors
+// (group ends follow the gate indices).
+Program split_kron_gates(const Program& in)
+{
+    Program out = in;
+    out.gates.clear();
+    std::vector<int> new_index(in.gates.size() + 1, 0);
+    for (size_t gi = 0; gi < in.gates.size(); gi++) {
+        new_index[gi] = (int)out.gates.size();
+        const PlanGate& g = in.gates[gi];
+        bool done = false;
+        if (g.id == 0 && g.k == 2) {
+            const double* m = in.mat.data() + 2 * g.moff;
+            // forms: M = K (ctl -1), M = P K (P after the factors), M = K P (P first)
+            for (int form = 0; form < 5 && !done; form++) {
+                const int ctl = form == 0 ? -1 : (form - 1) % 2;
+                const bool p_first = form >= 3;
+                double k4[32];
+                for (int r = 0; r < 4; r++)
+                    for (int c = 0; c < 4; c++) {
+                        // K = P M (P after) or K = M P (P first); P is an involution
+                        const int rr = (ctl >= 0 && !p_first) ? cnot_perm(ctl, r) : r;
+                        const int cc = (ctl >= 0 && p_first) ? cnot_perm(ctl, c) : c;
+                        k4[2 * (r * 4 + c)] = m[2 * (rr * 4 + cc)];
+                        k4[2 * (r * 4 + c) + 1] = m[2 * (rr * 4 + cc) + 1];
+                    }
+                double a[8], b[8];
+                if (!kron_factors(k4, a, b)) continue;
+                // a acts on index bit 1 (qubits[1]), b on bit 0 (qubits[0])
+                PlanGate cx{};
+                cx.id = 3;
+                cx.k = 2;
+                if (ctl >= 0) {
+                    cx.q[0] = g.q[1 - ctl];  // Kate's CNOT is {target, control}
+                    cx.q[1] = g.q[ctl];
+                    cx.mask = (1ull << cx.q[0]) | (1ull << cx.q[1]);
+                }
+                auto one = [&](const double* u, int q) {
+                    PlanGate h{};
+                    h.id = 0;
+                    h.k = 1;
+                    h.q[0] = q;
+                    h.mask = 1ull << q;
+                    h.moff = (int64_t)(out.mat.size() / 2);
+                    out.mat.insert(out.mat.end(), u, u + 8);
+                    out.gates.push_back(h);
+                };
+                if (ctl >= 0 && p_first) out.gates.push_back(cx);
+                one(b, g.q[0]);
+                one(a, g.q[1]);
+                if (ctl >= 0 && !p_first) out.gates.push_back(cx);
+                done = true;
+            }
+        }
+        if (!done) out.gates.push_back(g);
+    }
+    new_index[in.gates.size()] = (int)out.gates.size();
+    for (int& e : out.group_end) e = new_index[(size_t)e];
+    return out;
+}
+
 struct Builder {
     const Program& prog;
     const PlanOptions& opt;
@@ -1092,6 +1224,11 @@ static double run_cost(int run_bytes)
 
 std::string build_plan(const Program& prog, const PlanOptions& opt, Plan& plan)
 {
+    if (opt.split) {
+        PlanOptions o = opt;
+        o.split = false;
+        return build_plan(split_kron_gates(prog), o, plan);
+    }
     if (opt.auto_low) {
         // More low qubits per tile give longer HBM runs but fewer free tile
         // qubits, hence more passes; the light-cone packer's pass count is

@@ -139,6 +139,7 @@ struct PlanOptions {
     bool auto_low = false;   // build the plan for several low_qubits and keep the cheapest
     bool support = true;     // runs from |0...0> skip tiles known to be zero (used by the simulator)
     bool defer = true;       // execute() measures through the trailing permutation passes (simulator)
+    bool split = true;       // dense 2-qubit gates that are CNOT x (A (x) B) products run as their factors
 };
 
 struct Plan {

@@ -88,3 +88,62 @@ def test_brickwork_31q_swap_plan_matches_plain(gpu):
         assert abs(sim.norm() - 1.0) < 1e-12
     say("max |<Z> plain - <Z> swap plan|", np.max(np.abs(z_plain - z_swap)))
     assert np.max(np.abs(z_plain - z_swap)) < 1e-12
+
+
+def test_c2_28q_complex64_fused_forms(gpu):
+    """The 28-qubit depth-20 layered circuit: complex64 plain, and
+    GateMerger(5)-fused on the register path, on FP32 matrix-core windows
+    and on split-bf16 matrix-core windows, each against the complex128
+    state on the device: 1 - fidelity <= 1e-6 and max-abs amplitude error
+    (printed, bounded)."""
+    n = 28
+    gates = W.layered(n, 20, 2)
+    say("C2 28q complex64 forms")
+    with K.Simulator(K.Circuit(n, gates), dtype="c128") as ref:
+        ref.execute()
+        for label, fuse, mfma in (("plain", False, 0), ("fused valu", True, 0), ("fused mfma fp32", True, 2),
+                                  ("fused mfma bf16x3", True, 3)):
+            c = K.Circuit(n, gates).schedule().fuse(5) if fuse else K.Circuit(n, gates)
+            with K.Simulator(c, dtype="c64", mfma=mfma) as s32:
+                s32.execute()
+                ov, na, nb, mx = ref.compare(s32)
+                infid = 1.0 - abs(ov) ** 2 / (na * nb)
+                say(f"C2 {label}: 1-F {infid:.3e} max-abs {mx:.3e} norm {nb:.9f}")
+                assert abs(na - 1.0) < 1e-12
+                assert infid <= 1e-6, label
+                assert mx < 1e-6, label
+
+
+def test_c3_32q_complex64_brickwork(gpu):
+    """The 32-qubit complex64 brickwork (depth 24, long-range pairs every
+    4th layer, 32 GiB state): the plain program, and Kate's allocate(3)
+    swap plan (in-HBM remap passes), against the same circuit in complex128
+    (64 GiB) on the device: <Z_S> for every qubit and 8 Z strings, the
+    fidelity and the norm."""
+    n = 32
+    gates = W.brickwork(n, 24, 5)
+    masks = [1 << q for q in range(n)] + [int(m) for m in W.zstring_masks(n, 8, 3)]
+    say("C3 32q complex128 reference")
+    with K.Simulator(K.Circuit(n, gates), dtype="c128") as ref:
+        ref.execute()
+        z_ref = ref.expect_z(masks)
+        assert abs(ref.norm() - 1.0) < 1e-12
+        c_swap = K.Circuit(n, gates).schedule().group(8).allocate(3)
+        assert any(k == 0 for k, _ in c_swap.instructions())
+        for label, c in (("plain", K.Circuit(n, gates)), ("allocate(3) swap plan", c_swap)):
+            with K.Simulator(c, dtype="c64") as s32:
+                s32.execute()
+                if label != "plain":
+                    assert s32.timings()["remap_passes"] > 0
+                z = s32.expect_z(masks)
+                nrm = s32.norm()
+                zerr = float(np.max(np.abs(z - z_ref)))
+                msg = f"C3 32q complex64 {label}: max <Z_S> error {zerr:.3e} norm {nrm:.9f}"
+                if label == "plain":
+                    ov, na, nb, mx = ref.compare(s32)
+                    infid = 1.0 - abs(ov) ** 2 / (na * nb)
+                    msg += f" 1-F {infid:.3e} max-abs {mx:.3e}"
+                    assert infid <= 1e-6
+                say(msg)
+                assert zerr < 1e-5, label
+                assert abs(nrm - 1.0) < 1e-5, label

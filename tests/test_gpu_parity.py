@@ -157,6 +157,27 @@ def test_fused_programs_fp64(gpu):
     assert np.max(np.abs(st - O.run(compiled_from_circuit(c)))) <= TOL64
 
 
+@pytest.mark.parametrize("kind", ["layered", "brickwork", "mixed"])
+def test_split_fused_gates_match_oracle(gpu, kind):
+    """Fused programs whose dense 2-qubit gates the planner applies as
+    Kronecker factors and a layout CNOT: the state equals the oracle's run
+    of the same fused program (FP64, 1e-12) and the run with the split off
+    to rounding."""
+    say("split", kind)
+    n = 18
+    gates = {"layered": W.layered(n, 8, 7), "brickwork": W.brickwork(n, 6, 8, longrange_offset=9),
+             "mixed": W.random_mixed(n, 120, 9, max_dense=2) + W.layered(n, 3, 9)}[kind]
+    c = K.Circuit(n, gates).schedule().fuse(5)
+    ref = O.run(compiled_from_circuit(c))
+    with K.Simulator(c, dtype="c128") as a, K.Simulator(K.Circuit(n, gates).schedule().fuse(5), dtype="c128",
+                                                        split=False) as b:
+        a.execute()
+        b.execute()
+        sa, sb = a.state(real_order=False), b.state(real_order=False)
+    assert np.max(np.abs(sa - ref)) <= TOL64
+    assert np.max(np.abs(sa - sb)) <= 1e-13
+
+
 @pytest.mark.parametrize("n,fuse", [(20, 0), (20, 5), (16, 4)])
 def test_fp32_fidelity(gpu, n, fuse):
     say("fp32", n, fuse)

@@ -200,7 +200,7 @@ def test_plan_flags_validation_and_trailing_passes():
     import ctypes as C
     n = 22
     circ = K.Circuit(n, W.layered(n, 3, 42))
-    opt = K.Options(K.KATE_C128, 0, 0, 0, 0, 0, 0, 16)
+    opt = K.Options(K.KATE_C128, 0, 0, 0, 0, 0, 0, 32)
     p, r = C.c_int32(), C.c_int32()
     buf = C.create_string_buffer(1 << 16)
     rc = K.lib().kate_circuit_plan(circ._h, C.byref(opt), C.byref(p), C.byref(r), buf, 1 << 16)
@@ -225,3 +225,29 @@ def test_qcx_export_equals_kate(path, tmp_path):
     c.export_qcx(str(out))
     with open(path, "rb") as f:
         assert out.read_bytes() == f.read()
+
+
+def test_fused_two_qubit_gates_run_as_their_factors():
+    """GateMerger(5) on the layered circuit merges 1-qubit gates of two
+    neighbouring qubits with the CNOT between them into dense 4x4 gates; the
+    planner applies those as their Kronecker factors and a layout CNOT, so
+    the fused program plans like the unfused one (scheduled): same passes
+    and micro-steps, no dense 2-qubit register op left.  With the split off
+    the dense ops stay."""
+    import os
+    n = 20
+    gates = W.layered(n, 8, 3)
+    fused = K.Circuit(n, gates).schedule().fuse(5)
+    assert any(g.id == 0 and len(g.qubits) == 2 for g in fused.gates())
+    os.environ["KATE_DESCRIBE_OPS"] = "1"
+    try:
+        p_split, _, t_split = fused.plan("c64")
+        p_plain, _, t_plain = K.Circuit(n, gates).schedule().plan("c64")
+        p_nosplit, _, t_nosplit = fused.plan("c64", split=False)
+    finally:
+        os.environ.pop("KATE_DESCRIBE_OPS", None)
+    assert p_split == p_plain
+    dense2 = [f"dense:{m}" for m in (3, 5, 6, 9, 10, 12)]
+    assert not any(d + " " in t_split or d + "\n" in t_split for d in dense2)
+    assert any(d + " " in t_nosplit or d + "\n" in t_nosplit for d in dense2)
+    assert p_nosplit >= p_split
