@@ -122,7 +122,11 @@ enum {
        the CNOT between them) as dense 4x4 products instead of as their
        factors (1-qubit gates composed with their neighbours, the CNOT a
        layout update; same result to rounding) */
-    KATE_PLAN_NO_SPLIT = 16
+    KATE_PLAN_NO_SPLIT = 16,
+    /* plan the gates of a swap-free program only in the order given, instead
+       of also in a canonical order of the same dependency graph (ASAP level,
+       then lowest qubit) and keeping the cheaper plan */
+    KATE_PLAN_NO_REORDER = 32
 };
 
 /* Wall-clock split of the last execute, as Kate's execute(true) prints it

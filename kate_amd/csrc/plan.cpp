@@ -1222,12 +1222,70 @@ static double run_cost(int run_bytes)
     return 0.9;
 }
 
+// The same swap-free program with its gates in a canonical topological
+// order: by ASAP level (one more than the deepest earlier gate on a shared
+// qubit), then lowest qubit, then program order; one compute group.  Every
+// order of the same dependency graph (Kate's gateScheduling only reorders
+// within it, Circuit.hpp:12-98) maps to the same canonical order, so the
+// light-cone packer, whose greedy choices break ties by program order, no
+// longer depends on which topological order the caller passed.
+static Program canonical_order(const Program& in)
+{
+    Program out = in;
+    std::vector<int> level(64, 0);
+    std::vector<std::pair<std::pair<int, int>, int>> key;
+    key.reserve(in.gates.size());
+    for (size_t i = 0; i < in.gates.size(); i++) {
+        const PlanGate& g = in.gates[i];
+        int l = 0, lo = 64;
+        for (int j = 0; j < g.k; j++) {
+            l = std::max(l, level[(size_t)g.q[j]]);
+            lo = std::min(lo, g.q[j]);
+        }
+        for (int j = 0; j < g.k; j++) level[(size_t)g.q[j]] = l + 1;
+        key.push_back({{l + 1, lo}, (int)i});
+    }
+    std::sort(key.begin(), key.end());
+    out.gates.clear();
+    uint64_t mask = 0;
+    for (const auto& k : key) {
+        out.gates.push_back(in.gates[(size_t)k.second]);
+        mask |= in.gates[(size_t)k.second].mask;
+    }
+    out.group_end.assign(1, (int)out.gates.size());
+    out.group_mask.assign(1, mask);
+    out.instr_kind.assign(1, 1);
+    out.instr_pairs.assign(1, std::vector<int>());
+    return out;
+}
+
+// Relative cost of a plan: passes x the HBM cost of its run length, plus a
+// tenth of a pass per micro-step.
+static double plan_cost(const Plan& p, int amp_bytes)
+{
+    const int run = (1 << std::min(p.low_qubits, p.nq_int)) * amp_bytes;
+    return (double)p.passes.size() * run_cost(run) + 0.1 * (double)p.micro.size();
+}
+
 std::string build_plan(const Program& prog, const PlanOptions& opt, Plan& plan)
 {
     if (opt.split) {
         PlanOptions o = opt;
         o.split = false;
         return build_plan(split_kron_gates(prog), o, plan);
+    }
+    if (opt.reorder) {
+        PlanOptions o = opt;
+        o.reorder = false;
+        std::string e = build_plan(prog, o, plan);
+        bool swap_free = !prog.gates.empty();
+        for (int k : prog.instr_kind) swap_free = swap_free && k == 1;
+        if (!e.empty() || !swap_free) return e;
+        Plan alt;
+        if (build_plan(canonical_order(prog), o, alt).empty() &&
+            plan_cost(alt, opt.amp_bytes) < plan_cost(plan, opt.amp_bytes))
+            plan = std::move(alt);
+        return "";
     }
     if (opt.auto_low) {
         // More low qubits per tile give longer HBM runs but fewer free tile
@@ -1251,8 +1309,7 @@ std::string build_plan(const Program& prog, const PlanOptions& opt, Plan& plan)
                 if (first_err.empty()) first_err = e;
                 continue;
             }
-            const int run = (1 << std::min(low, cand.nq_int)) * opt.amp_bytes;
-            const double cost = (double)cand.passes.size() * run_cost(run) + 0.1 * (double)cand.micro.size();
+            const double cost = plan_cost(cand, opt.amp_bytes);
             if (!have || cost < best) {
                 best = cost;
                 plan = std::move(cand);

@@ -140,6 +140,7 @@ struct PlanOptions {
     bool support = true;     // runs from |0...0> skip tiles known to be zero (used by the simulator)
     bool defer = true;       // execute() measures through the trailing permutation passes (simulator)
     bool split = true;       // dense 2-qubit gates that are CNOT x (A (x) B) products run as their factors
+    bool reorder = true;     // swap-free programs: also plan the gates in a canonical order, keep the cheaper
 };
 
 struct Plan {

@@ -349,9 +349,11 @@ std::string plan_options(const kate_options& opt, kate::PlanOptions& po)
     po.micro_bits = opt.micro_bits ? opt.micro_bits : 4;
     po.amp_bytes = opt.dtype == KATE_C128 ? 16 : 8;
     if (opt.plan_flags &
-        ~(KATE_PLAN_NO_EULER | KATE_PLAN_NO_DIRECT | KATE_PLAN_NO_SUPPORT | KATE_PLAN_NO_DEFER | KATE_PLAN_NO_SPLIT))
+        ~(KATE_PLAN_NO_EULER | KATE_PLAN_NO_DIRECT | KATE_PLAN_NO_SUPPORT | KATE_PLAN_NO_DEFER | KATE_PLAN_NO_SPLIT |
+          KATE_PLAN_NO_REORDER))
         return "unknown plan_flags bits";
     po.split = !(opt.plan_flags & KATE_PLAN_NO_SPLIT);
+    po.reorder = !(opt.plan_flags & KATE_PLAN_NO_REORDER);
     po.defer = !(opt.plan_flags & KATE_PLAN_NO_DEFER);
     po.euler = !(opt.plan_flags & KATE_PLAN_NO_EULER);
     po.direct = !(opt.plan_flags & KATE_PLAN_NO_DIRECT);

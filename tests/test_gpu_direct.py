@@ -70,10 +70,10 @@ def test_direct_single_step_pass_at_28_qubits(gpu):
     n = 28
     gates = W.layered(n, 20, 2)
     circ = gates + inverse(gates)
-    _, _, text = K.Circuit(n, circ).plan("c128", low_qubits=3)
+    _, _, text = K.Circuit(n, circ).plan("c128", low_qubits=3, reorder=False)
     assert any(" micro=1 " in ln and "direct=first,last" in ln for ln in text.splitlines())
     with K.Simulator(K.Circuit(n, []), dtype="c128") as zero, \
-            K.Simulator(K.Circuit(n, circ), dtype="c128", low_qubits=3) as a:
+            K.Simulator(K.Circuit(n, circ), dtype="c128", low_qubits=3, reorder=False) as a:
         zero.init_basis(0)
         a.execute()
         _, na, nb, mx = zero.compare(a)
@@ -153,7 +153,10 @@ def test_runtime_savings_match_the_plain_run(gpu, seed):
     """Random programs (all gate kinds, some compiled with Kate's swap
     allocator): execute() with the direct steps, the support skipping and
     the deferred trailing passes gives the same proba_state and the same
-    state, bit for bit, as with all three switched off."""
+    state as with all three switched off: the support skipping and the
+    deferral bit for bit; the direct first / last steps compile the same
+    register ops a second time, whose FMA contraction can differ in the last
+    bit (observed 2e-18 on amplitudes, 2e-15 on proba angles)."""
     rng = np.random.default_rng(100 + seed)
     n = int(rng.integers(13, 21))
     kind = seed % 3
@@ -169,9 +172,13 @@ def test_runtime_savings_match_the_plain_run(gpu, seed):
             c = K.Circuit(n, gates)
             return c.schedule().group(8).allocate(alloc) if alloc else c
         with K.Simulator(mk(), dtype="c128", defer=True) as a, \
-                K.Simulator(mk(), dtype="c128", direct=False, support=False, defer=False) as b:
+                K.Simulator(mk(), dtype="c128", direct=False, support=False, defer=False) as b, \
+                K.Simulator(mk(), dtype="c128", direct=False, defer=True) as c:
             pa = a.execute()
             pb = b.execute()
-            sa, sb = a.state(), b.state()
-        assert np.array_equal(pa, pb), (seed, alloc)
-        assert np.array_equal(sa, sb), (seed, alloc)
+            pc = c.execute()
+            sa, sb, sc = a.state(), b.state(), c.state()
+        assert np.array_equal(pc, pb), (seed, alloc)
+        assert np.array_equal(sc, sb), (seed, alloc)
+        assert np.max(np.abs(pa - pb)) <= 1e-13, (seed, alloc)
+        assert np.max(np.abs(sa - sb)) <= 1e-15, (seed, alloc)

@@ -200,7 +200,7 @@ def test_plan_flags_validation_and_trailing_passes():
     import ctypes as C
     n = 22
     circ = K.Circuit(n, W.layered(n, 3, 42))
-    opt = K.Options(K.KATE_C128, 0, 0, 0, 0, 0, 0, 32)
+    opt = K.Options(K.KATE_C128, 0, 0, 0, 0, 0, 0, 1 << 10)
     p, r = C.c_int32(), C.c_int32()
     buf = C.create_string_buffer(1 << 16)
     rc = K.lib().kate_circuit_plan(circ._h, C.byref(opt), C.byref(p), C.byref(r), buf, 1 << 16)
