@@ -180,6 +180,18 @@ int kate_sim_execute(kate_sim* sim, const double* theta_phi_in, double* theta_ph
 /* No counterpart in Kate: <Z_S> = sum_x |a_x|^2 (-1)^popcount(x & S) for each
  * real-qubit mask S, accumulated in FP64, normalised by the state's norm. */
 int kate_sim_expect_z(kate_sim* sim, const uint64_t* masks, int32_t count, double* out);
+/* No counterpart in Kate, and no allocating counterpart either: the
+ * kate_sim_expect_z reduction for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  d_state: a caller-owned device state of 2^nqubits amplitudes
+ * (dtype, virtual order, 12 <= nqubits <= 40) on `device`; masks: `count`
+ * (1..15) Z strings over state bits (host array); d_out: `count` doubles of
+ * device memory receiving <Z_S> = sum_x |a_x|^2 (-1)^popcount(x & S) / norm,
+ * FP64.  Enqueued on `stream` (asynchronous).  The partial sums live in a
+ * fixed 256 KiB device array that the library holds on each GPU until the
+ * process ends; calls that overlap on one GPU share it and race. */
+int kate_state_expect_z_noalloc(int32_t device, int32_t dtype, const void* d_state, int32_t nqubits,
+                                const uint64_t* masks, int32_t count, double* d_out, void* stream);
 /* No counterpart in Kate: sum_x |a_x|^2 in FP64. */
 int kate_sim_norm(kate_sim* sim, double* out);
 /* No counterpart in Kate: rescale the state to norm 1 (factor in FP64). */

@@ -74,7 +74,7 @@ EXPORTS = [
     "kate_sim_measure_proba", "kate_sim_execute", "kate_sim_expect_z", "kate_sim_norm", "kate_sim_renormalize",
     "kate_sim_read_state", "kate_sim_write_state", "kate_sim_device_state", "kate_sim_last_timings",
     "kate_sim_synchronize", "kate_sim_stream", "kate_sim_describe", "kate_plan_describe", "kate_sim_run_timed",
-    "kate_sim_overlap", "kate_sim_execute_timed",
+    "kate_sim_overlap", "kate_sim_execute_timed", "kate_state_expect_z_noalloc",
     "kate_circuit_create", "kate_circuit_destroy", "kate_circuit_append", "kate_circuit_schedule",
     "kate_circuit_fuse", "kate_circuit_group", "kate_circuit_allocate", "kate_circuit_compile_default",
     "kate_circuit_compile_opt", "kate_circuit_export_qcx", "kate_circuit_info", "kate_circuit_gate",
@@ -139,6 +139,7 @@ def lib():
         L.kate_sim_overlap.argtypes = [VP, VP, DP]
         L.kate_sim_run_timed.argtypes = [VP, C.POINTER(C.c_float), I32P, C.c_int32, I32P]
         L.kate_sim_execute_timed.argtypes = [VP, DP, DP, C.POINTER(C.c_float), U64P, C.c_int32, I32P]
+        L.kate_state_expect_z_noalloc.argtypes = [C.c_int32, C.c_int32, VP, C.c_int32, U64P, C.c_int32, VP, VP]
         L.kate_plan_describe.argtypes = [VP, C.POINTER(Options), I32P, I32P, C.c_char_p, C.c_size_t]
         L.kate_circuit_plan.argtypes = [VP, C.POINTER(Options), I32P, I32P, C.c_char_p, C.c_size_t]
         L.kate_batch_create.argtypes = [VP, C.c_int32, C.POINTER(Options), C.POINTER(VP)]
