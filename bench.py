@@ -197,6 +197,12 @@ def main():
     circuit_gbs = circ_bytes / (float(pass_ms.sum()) * 1e-3) / 1e9
     log(f"fp64: {ms64:.2f} ms/circuit, {len(pass_ms)} passes ({int(full.sum())} over every tile), "
         f"mean full pass {mean_pass:.3f} ms = {achieved:.0f} GB/s")
+    # the HBM time each pass's access pattern allows: a plain in-place copy
+    # of the same tiles (no arithmetic, no LDS), timed the same way
+    ceil_ms = sim.copy_ceiling()
+    ceil_full = float(ceil_ms[full].mean())
+    log(f"fp64 copy ceiling of the same tiles: mean {ceil_full:.3f} ms = {bpp / (ceil_full * 1e-3) / 1e9:.0f} GB/s; "
+        f"the full passes reach {ceil_full / mean_pass:.3f} of it")
     flops = plan_flops(c, "c128", max(n, 12))
     full_flops = float(sum(f for f, on in zip(flops, full) if on))
     tflops = full_flops / (float(pass_ms[full].sum()) * 1e-3) / 1e12
@@ -228,6 +234,11 @@ def main():
                      "mean_launch_ms": round(mean_pass, 4), "launches": int(full.sum()),
                      "run": "the headline execute(): its passes that launch every tile (the others skip "
                             "tiles that are still zero)",
+                     "copy_ceiling": {"mean_launch_ms": round(ceil_full, 4),
+                                      "achieved": round(bpp / (ceil_full * 1e-3) / 1e9, 1),
+                                      "frac_of_ceiling": round(ceil_full / mean_pass, 4),
+                                      "what": "the same passes' tiles copied in place with no arithmetic "
+                                              "(kate_sim_copy_ceiling): the HBM rate their access pattern allows"},
                      "compute": {"bound": "fp64 valu", "achieved": round(tflops, 2),
                                  "peak": FP64_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
                                  "frac": round(tflops / FP64_VALU_PEAK_TFLOPS, 4),

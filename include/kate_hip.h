@@ -221,6 +221,12 @@ int kate_sim_stream(kate_sim* sim, void** stream);
  * whose execute(true) only prints per-group wall times
  * (HIPcode/simulator.hpp:322-351). */
 int kate_sim_run_timed(kate_sim* sim, float* pass_ms, int32_t* pass_is_remap, int32_t cap, int32_t* npasses);
+/* Calibration for the passes' roofline (no counterpart in Kate): for each
+ * pass of the plan, the time of a plain in-place copy of every tile with the
+ * pass's tile positions (each amplitude read once and written once, no
+ * arithmetic, no LDS), i.e. the HBM time the pass's access pattern allows.
+ * The state is left unchanged.  pass_ms[i] for i < cap; *npasses passes. */
+int kate_sim_copy_ceiling(kate_sim* sim, float* pass_ms, int32_t cap, int32_t* npasses);
 /* kate_sim_execute with a HIP event after every pass on the simulator's
  * stream: pass_ms[i] is pass i's time and pass_tiles[i] the number of tiles
  * it launched (fewer than 2^(n - tile qubits) when the run from |0...0>

@@ -74,7 +74,7 @@ EXPORTS = [
     "kate_sim_measure_proba", "kate_sim_execute", "kate_sim_expect_z", "kate_sim_norm", "kate_sim_renormalize",
     "kate_sim_read_state", "kate_sim_write_state", "kate_sim_device_state", "kate_sim_last_timings",
     "kate_sim_synchronize", "kate_sim_stream", "kate_sim_describe", "kate_plan_describe", "kate_sim_run_timed",
-    "kate_sim_overlap", "kate_sim_execute_timed", "kate_state_expect_z_noalloc",
+    "kate_sim_overlap", "kate_sim_execute_timed", "kate_sim_copy_ceiling", "kate_state_expect_z_noalloc",
     "kate_circuit_create", "kate_circuit_destroy", "kate_circuit_append", "kate_circuit_schedule",
     "kate_circuit_fuse", "kate_circuit_group", "kate_circuit_allocate", "kate_circuit_compile_default",
     "kate_circuit_compile_opt", "kate_circuit_export_qcx", "kate_circuit_info", "kate_circuit_gate",
@@ -139,6 +139,7 @@ def lib():
         L.kate_sim_overlap.argtypes = [VP, VP, DP]
         L.kate_sim_run_timed.argtypes = [VP, C.POINTER(C.c_float), I32P, C.c_int32, I32P]
         L.kate_sim_execute_timed.argtypes = [VP, DP, DP, C.POINTER(C.c_float), U64P, C.c_int32, I32P]
+        L.kate_sim_copy_ceiling.argtypes = [VP, C.POINTER(C.c_float), C.c_int32, I32P]
         L.kate_state_expect_z_noalloc.argtypes = [C.c_int32, C.c_int32, VP, C.c_int32, U64P, C.c_int32, VP, VP]
         L.kate_plan_describe.argtypes = [VP, C.POINTER(Options), I32P, I32P, C.c_char_p, C.c_size_t]
         L.kate_circuit_plan.argtypes = [VP, C.POINTER(Options), I32P, I32P, C.c_char_p, C.c_size_t]
@@ -382,6 +383,16 @@ class Simulator:
                                             ms.ctypes.data_as(C.POINTER(C.c_float)), _p(tiles, U64P), cap,
                                             C.byref(n)), "execute_timed")
         return out.reshape(self.n, 2), ms[:n.value].astype(np.float64), tiles[:n.value].copy()
+
+    def copy_ceiling(self) -> np.ndarray:
+        """Per pass: ms of a plain in-place copy of its tiles (same tile
+        positions, no arithmetic): the HBM time its access pattern allows."""
+        cap = 1 << 16
+        ms = np.zeros(cap, dtype=np.float32)
+        n = C.c_int32()
+        _check(lib().kate_sim_copy_ceiling(self._h, ms.ctypes.data_as(C.POINTER(C.c_float)), cap, C.byref(n)),
+               "copy_ceiling")
+        return ms[:n.value].astype(np.float64)
 
     def compare(self, other: "Simulator") -> Tuple[complex, float, float, float]:
         """(<self|other>, <self|self>, <other|other>, max |self_x - other_x|),

@@ -328,6 +328,46 @@ __global__ __launch_bounds__(256) void k_rowsum(const double* __restrict__ rows,
     }
 }
 
+// Calibration: the pass's tiles read and written back in place with no
+// arithmetic (every amplitude once each way, the load positions of the
+// pass, 2^(TB-4) threads of 16 registers, no LDS).  `one` is 1 at run time,
+// so the stores cannot be elided.
+template <typename R, int TB>
+__global__ __launch_bounds__(1 << (TB - 4)) void k_copy_tile(Cx<R>* __restrict__ state, PassArgs pa, R one)
+{
+    constexpr int NB = 4;
+    const uint32_t tid = threadIdx.x;
+    uint64_t tb = rest_base(pa, blockIdx.x);
+#pragma unroll
+    for (int b = 0; b < TB - NB; b++) tb |= (uint64_t)((tid >> b) & 1u) << pa.load_pos[b];
+    uint64_t off[1 << NB];
+    Cx<R> v[1 << NB];
+#pragma unroll
+    for (int i = 0; i < (1 << NB); i++) {
+        uint64_t o = tb;
+#pragma unroll
+        for (int k = 0; k < NB; k++)
+            if ((i >> k) & 1) o |= 1ull << pa.load_pos[TB - NB + k];
+        off[i] = o;
+        v[i] = state[o];
+    }
+#pragma unroll
+    for (int i = 0; i < (1 << NB); i++) state[off[i]] = Cx<R>{v[i].re * one, v[i].im * one};
+}
+
+template <typename R>
+int copy_tile_r(int tile_bits, void* state, const PassArgs& pa, hipStream_t s)
+{
+    const unsigned grid = 1u << pa.nrest;
+    switch (tile_bits) {
+#define CT(TB) case TB: hipLaunchKernelGGL((k_copy_tile<R, TB>), dim3(grid), dim3(1 << (TB - 4)), 0, s, (Cx<R>*)state, pa, (R)1); break;
+        CT(10) CT(11) CT(12) CT(13)
+#undef CT
+    default: return (int)hipErrorInvalidValue;
+    }
+    return (int)hipGetLastError();
+}
+
 inline unsigned grid_for(uint64_t n)
 {
     const uint64_t b = (n + 255) / 256;
@@ -401,6 +441,14 @@ int launch_pass(int dtype, int tile_bits, void* state, int nq_int, const PassArg
         return (int)hipErrorInvalidValue;
     if (dtype == 0) return launch_pass_f64(tile_bits, state, nq_int, pa, dp, stream);
     return launch_pass_f32(tile_bits, state, nq_int, pa, dp, stream);
+}
+
+int launch_copy_tiles(int dtype, int tile_bits, void* state, int nq_int, const PassArgs& pa, void* stream)
+{
+    if (pa.tile_bits != tile_bits || nq_int < tile_bits || nq_int - tile_bits > 31 || pa.nrest > nq_int - tile_bits)
+        return (int)hipErrorInvalidValue;
+    if (dtype == 0) return copy_tile_r<double>(tile_bits, state, pa, (hipStream_t)stream);
+    return copy_tile_r<float>(tile_bits, state, pa, (hipStream_t)stream);
 }
 
 int launch_init_basis(int dtype, void* state, int nq_int, uint64_t index, void* stream)

@@ -58,6 +58,9 @@ int launch_batch(int dtype, const BatchArgs& a, void* stream);
 // One pass (gate application or remap) over the state.
 int launch_pass(int dtype, int tile_bits, void* state, int nq_int, const PassArgs& pa, const DevPlan& dp,
                 void* stream);
+// Calibration: every tile of the pass read and written back in place
+// (load positions, no arithmetic, no LDS): the HBM time of its access pattern.
+int launch_copy_tiles(int dtype, int tile_bits, void* state, int nq_int, const PassArgs& pa, void* stream);
 // |index> (basis state).
 int launch_init_basis(int dtype, void* state, int nq_int, uint64_t index, void* stream);
 // Product state: factor[b*nq + j] (complex128) for bit value b of qubit j.

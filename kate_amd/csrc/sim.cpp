@@ -597,6 +597,35 @@ int kate_sim_run_timed(kate_sim* s, float* pass_ms, int32_t* pass_is_remap, int3
     return rc;
 }
 
+int kate_sim_copy_ceiling(kate_sim* s, float* pass_ms, int32_t cap, int32_t* npasses)
+{
+    if (!s || cap < 0) return fail(KATE_ERR_ARG, "bad arguments");
+    KATE_HIP(hipSetDevice(s->device));
+    if (int rc0 = ensure_final(s)) return rc0;
+    const size_t np = s->plan.passes.size();
+    if (npasses) *npasses = (int32_t)np;
+    std::vector<hipEvent_t> ev(np + 1, nullptr);
+    int rc = KATE_OK;
+    for (auto& e : ev)
+        if (rc == KATE_OK && hipEventCreate(&e) != hipSuccess) rc = fail(KATE_ERR_HIP, "hipEventCreate");
+    if (rc == KATE_OK && hipEventRecord(ev[0], s->stream) != hipSuccess) rc = fail(KATE_ERR_HIP, "hipEventRecord");
+    for (size_t i = 0; i < np && rc == KATE_OK; i++) {
+        int e = kate::launch_copy_tiles(s->dtype, s->plan.tile_bits, s->d_state, s->nq_int, s->plan.passes[i],
+                                        s->stream);
+        if (e) rc = fail(KATE_ERR_HIP, std::string("copy launch: ") + hipGetErrorString((hipError_t)e));
+        else if (hipEventRecord(ev[i + 1], s->stream) != hipSuccess) rc = fail(KATE_ERR_HIP, "hipEventRecord");
+    }
+    if (rc == KATE_OK && hipEventSynchronize(ev[np]) != hipSuccess) rc = fail(KATE_ERR_HIP, "hipEventSynchronize");
+    for (size_t i = 0; i < np && rc == KATE_OK; i++) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) != hipSuccess) rc = fail(KATE_ERR_HIP, "hipEventElapsedTime");
+        if ((int32_t)i < cap && pass_ms) pass_ms[i] = ms;
+    }
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    return rc;
+}
+
 int kate_sim_measure_proba(kate_sim* s, double* tp)
 {
     if (!s || !tp) return fail(KATE_ERR_ARG, "null argument");

@@ -182,3 +182,20 @@ def test_runtime_savings_match_the_plain_run(gpu, seed):
         assert np.array_equal(sc, sb), (seed, alloc)
         assert np.max(np.abs(pa - pb)) <= 1e-13, (seed, alloc)
         assert np.max(np.abs(sa - sb)) <= 1e-15, (seed, alloc)
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_copy_ceiling_leaves_the_state_unchanged(gpu, dtype):
+    """The copy-ceiling calibration copies every tile of every pass in place:
+    one time per pass, and the state is bit for bit what it was."""
+    say("copy ceiling", dtype)
+    n = 18
+    gates = W.layered(n, 4, 3)
+    with K.Simulator(K.Circuit(n, gates), dtype=dtype) as sim:
+        sim.execute()
+        before = sim.state()
+        ms = sim.copy_ceiling()
+        after = sim.state()
+        passes = sim.timings()["passes"]
+    assert len(ms) == passes and np.all(ms > 0)
+    assert np.array_equal(before, after)
