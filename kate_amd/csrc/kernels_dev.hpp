@@ -283,6 +283,37 @@ __device__ __forceinline__ void apply_op(Cx<R> (&v)[N], uint32_t code, const Cx<
     }
 }
 
+// The op set of passes the planner marks light (PassArgs::light): Euler-form
+// Kronecker ops without a right diagonal, dense 1-qubit ops, phases,
+// Hadamards and CNOTs -- what RX/RZ layers with CNOT ladders lower to.  A
+// shorter decision tree than apply_op, and no multi-qubit dense op holding
+// its inputs and outputs at once.
+template <typename R, int N>
+__device__ __forceinline__ void apply_op_light(Cx<R> (&v)[N], uint32_t code, const Cx<R>* __restrict__ cf)
+{
+    switch (code) {
+#define D1(M) \
+    case KATE_OPC(OK_DENSE, M): if constexpr ((M) < N) op_dense<M>(v, cf); break;
+        D1(1) D1(2) D1(4) D1(8)
+#undef D1
+#define E(M) \
+    case KATE_OPC(OK_EKRON, M): if constexpr ((M) < N) op_ekron<M, false>(v, cf); break;
+        E(3) E(5) E(6) E(7) E(9) E(10) E(11) E(12) E(13) E(14) E(15)
+#undef E
+#define P(J) \
+    case KATE_OPC(OK_PHASE, J): if constexpr ((1 << (J)) < N) op_phase<J>(v, cf); break; \
+    case KATE_OPC(OK_H, J): if constexpr ((1 << (J)) < N) op_h<J>(v); break;
+        P(0) P(1) P(2) P(3)
+#undef P
+#define CN(T, C) \
+    case KATE_OPC(OK_CNOT, (T) * 4 + (C)): if constexpr ((1 << (T)) < N && (1 << (C)) < N) op_cnot<T, C>(v); break;
+        CN(0, 1) CN(0, 2) CN(0, 3) CN(1, 0) CN(1, 2) CN(1, 3) CN(2, 0) CN(2, 1) CN(2, 3) CN(3, 0) CN(3, 1) CN(3, 2)
+#undef CN
+    default:
+        break;
+    }
+}
+
 // LDS index of tile index y under a layout (see MicroStep::cols).
 template <int TB>
 __device__ __forceinline__ uint32_t layout(uint32_t y, const uint16_t* __restrict__ cols)
@@ -556,13 +587,22 @@ __device__ __forceinline__ uint32_t reg_offset(int s, const uint32_t (&o)[4])
 // 32-bit words: a 16-bit field would be a vector load (there are no 16-bit
 // scalar loads), whose wait also drains every other vector memory operation
 // of the wave.
-template <typename R, int N>
+template <typename R, int N, bool LIGHT>
 __device__ __forceinline__ void reg_step_ops(Cx<R> (&v)[N], const MicroStep* __restrict__ msp, uint32_t op_begin,
                                              uint32_t op_count, const MicroOp* __restrict__ ops,
                                              const Cx<R>* __restrict__ coef)
 {
     const uint32_t w0 = reinterpret_cast<const uint32_t*>(msp)[0];
     const uint32_t code0 = w0 >> 16, coef0 = reinterpret_cast<const uint32_t*>(msp)[3];
+    // Light complex128 passes: the common step (RX/RZ layers), one Kronecker
+    // op in Euler form on every register bit, is taken before the op switch
+    // (in the complex64 kernels the extra path would cost occupancy).
+    if constexpr (LIGHT && sizeof(R) == 8) {
+        if (op_count == 1 && code0 == KATE_OPC(OK_EKRON, N - 1)) {
+            op_ekron<N - 1, false>(v, coef + coef0);
+            return;
+        }
+    }
     for (uint32_t o = 0; o < op_count; o++) {
         uint32_t code = code0, cofs = coef0;
         if (o) {
@@ -570,7 +610,8 @@ __device__ __forceinline__ void reg_step_ops(Cx<R> (&v)[N], const MicroStep* __r
             code = ow[0] & 0xFFFFu;
             cofs = ow[1];
         }
-        apply_op<R>(v, code, coef + cofs);
+        if constexpr (LIGHT) apply_op_light<R>(v, code, coef + cofs);
+        else apply_op<R>(v, code, coef + cofs);
     }
 }
 
@@ -612,7 +653,7 @@ __device__ __forceinline__ uint64_t direct_reg_offset(const uint64_t* __restrict
     return off;
 }
 
-template <typename R, int TB, bool WIDE, int NB>
+template <typename R, int TB, bool WIDE, int NB, bool LIGHT>
 __device__ __forceinline__ void run_steps(const PassArgs& pa, Cx<R>* tile, const MicroStep* __restrict__ micro,
                                           const MicroOp* __restrict__ ops, const Cx<R>* __restrict__ coef,
                                           const uint32_t* __restrict__ raw, uint32_t m_begin, uint32_t m_end)
@@ -627,7 +668,7 @@ __device__ __forceinline__ void run_steps(const PassArgs& pa, Cx<R>* tile, const
             Cx<R> v[N];
 #pragma unroll
             for (int s = 0; s < N; s++) v[s] = lds_at(tile, b ^ reg_offset<NB>(s, oc));
-            reg_step_ops<R, N>(v, msp, op_begin, op_count, ops, coef);
+            reg_step_ops<R, N, LIGHT>(v, msp, op_begin, op_count, ops, coef);
 #pragma unroll
             for (int s = 0; s < N; s++) lds_at(tile, b ^ reg_offset<NB>(s, oc)) = v[s];
         } else if (WIDE) {
@@ -690,7 +731,7 @@ __device__ __forceinline__ void store_tile(Cx<R>* __restrict__ state, const Pass
 // (its bits are the non-tile qubits rest_pos[]).  Each of the 2^(TB-NB)
 // threads holds N = 2^NB amplitudes; the register bits are tile bits
 // TB-NB..TB-1 when loading and storing.
-template <typename R, int TB, bool WIDE, int NB>
+template <typename R, int TB, bool WIDE, int NB, bool LIGHT>
 __device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassArgs& pa, uint64_t blk, Cx<R>* tile,
                                           const MicroStep* __restrict__ micro, const MicroOp* __restrict__ ops,
                                           const Cx<R>* __restrict__ coef, const uint32_t* __restrict__ raw)
@@ -719,7 +760,7 @@ __device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassA
 #pragma unroll
             for (int s = 0; s < N; s++) v[s] = state[gt ^ direct_reg_offset<NB>(pa.first_g, s)];
         }
-        reg_step_ops<R, N>(v, msp, op_begin, op_count, ops, coef);
+        reg_step_ops<R, N, LIGHT>(v, msp, op_begin, op_count, ops, coef);
         m_begin = 1;
         if (pa.micro_count == 1 && pa.fuse_last) {
             // the store of one thread can hit an amplitude that another
@@ -777,7 +818,7 @@ __device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassA
     __syncthreads();
 
     if (pa.fuse_last) m_end--;
-    run_steps<R, TB, WIDE, NB>(pa, tile, micro, ops, coef, raw, m_begin, m_end);
+    run_steps<R, TB, WIDE, NB, LIGHT>(pa, tile, micro, ops, coef, raw, m_begin, m_end);
     if (pa.fuse_last) {
         // direct last step: LDS in the step's layout -> registers -> ops -> HBM
         const MicroStep* __restrict__ msp = micro + pa.micro_begin + m_end;
@@ -786,7 +827,7 @@ __device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassA
         Cx<R> v[N];
 #pragma unroll
         for (int s = 0; s < N; s++) v[s] = lds_at(tile, b ^ reg_offset<NB>(s, oc));
-        reg_step_ops<R, N>(v, msp, op_begin, op_count, ops, coef);
+        reg_step_ops<R, N, LIGHT>(v, msp, op_begin, op_count, ops, coef);
         const uint64_t gs = base ^ direct_thread_offset<TB, NB>(pa.last_g, tid);
 #pragma unroll
         for (int s = 0; s < N; s++) state[gs ^ direct_reg_offset<NB>(pa.last_g, s)] = v[s];
@@ -795,15 +836,24 @@ __device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassA
     }
 }
 
-template <typename R, int TB, bool WIDE, int NB>
-__global__ __launch_bounds__(1 << (TB - NB)) void k_pass(Cx<R>* __restrict__ state, PassArgs pa,
+// Minimum waves per SIMD the register allocation must allow: two for
+// complex128 16-amplitude threads (their 64 KiB LDS tiles allow two per CU),
+// four for the other light variants (as many as their LDS tiles allow);
+// the compiler's own choice otherwise.
+template <typename R, int NB, bool LIGHT>
+constexpr int kPassWavesPerEU = (sizeof(R) == 8 && NB == 4) ? 2 : (LIGHT ? 4 : 1);
+
+// LIGHT: the pass's ops are all in the light set (apply_op_light).
+template <typename R, int TB, bool WIDE, int NB, bool LIGHT>
+__global__ __launch_bounds__(1 << (TB - NB)) __attribute__((amdgpu_waves_per_eu(kPassWavesPerEU<R, NB, LIGHT>)))
+void k_pass(Cx<R>* __restrict__ state, PassArgs pa,
                                                          const MicroStep* __restrict__ micro,
                                                          const MicroOp* __restrict__ ops,
                                                          const Cx<R>* __restrict__ coef,
                                                          const uint32_t* __restrict__ raw)
 {
     __shared__ Cx<R> tile[1 << TB];
-    tile_pass<R, TB, WIDE, NB>(state, pa, blockIdx.x, tile, micro, ops, coef, raw);
+    tile_pass<R, TB, WIDE, NB, LIGHT>(state, pa, blockIdx.x, tile, micro, ops, coef, raw);
 }
 
 // ---- many circuits in one persistent launch ---------------------------------
@@ -879,7 +929,7 @@ void k_batch(Cx<R>* __restrict__ states,
             }
             __syncthreads();
         }
-        tile_pass<R, TB, WIDE, NB>(states + circ_off[task.circuit], passes[task.pass], task.tile, tile, micro, ops,
+        tile_pass<R, TB, WIDE, NB, false>(states + circ_off[task.circuit], passes[task.pass], task.tile, tile, micro, ops,
                                    coef, raw);
         // every wave's stores of this tile have completed (each wave drains
         // its memory counters; a workgroup-scope barrier alone does not wait
@@ -900,11 +950,14 @@ int pass_t(void* state, int nq_int, const PassArgs& pa, const DevPlan& dp, hipSt
     (void)nq_int;
     const unsigned grid = 1u << pa.nrest;
     if (pa.has_wide)
-        hipLaunchKernelGGL((k_pass<R, TB, true, NB>), dim3(grid), dim3(1 << (TB - NB)), 0, s, (Cx<R>*)state, pa,
-                           dp.micro, dp.ops, (const Cx<R>*)dp.coef, dp.raw);
+        hipLaunchKernelGGL((k_pass<R, TB, true, NB, false>), dim3(grid), dim3(1 << (TB - NB)), 0, s, (Cx<R>*)state,
+                           pa, dp.micro, dp.ops, (const Cx<R>*)dp.coef, dp.raw);
+    else if (pa.light)
+        hipLaunchKernelGGL((k_pass<R, TB, false, NB, true>), dim3(grid), dim3(1 << (TB - NB)), 0, s, (Cx<R>*)state,
+                           pa, dp.micro, dp.ops, (const Cx<R>*)dp.coef, dp.raw);
     else
-        hipLaunchKernelGGL((k_pass<R, TB, false, NB>), dim3(grid), dim3(1 << (TB - NB)), 0, s, (Cx<R>*)state, pa,
-                           dp.micro, dp.ops, (const Cx<R>*)dp.coef, dp.raw);
+        hipLaunchKernelGGL((k_pass<R, TB, false, NB, false>), dim3(grid), dim3(1 << (TB - NB)), 0, s, (Cx<R>*)state,
+                           pa, dp.micro, dp.ops, (const Cx<R>*)dp.coef, dp.raw);
     return (int)hipGetLastError();
 }
 

@@ -993,10 +993,32 @@ struct Builder {
         fill_store_bytes(pa);
         pa.micro_count = (uint32_t)(plan.micro.size() - pa.micro_begin);
         pa.has_wide = 0;
-        for (uint32_t m = pa.micro_begin; m < plan.micro.size(); m++)
-            if (plan.micro[m].kind != SK_REG) pa.has_wide = 1;
+        pa.light = 1;
+        for (uint32_t m = pa.micro_begin; m < plan.micro.size(); m++) {
+            const MicroStep& ms = plan.micro[m];
+            if (ms.kind != SK_REG) pa.has_wide = 1;
+            for (uint32_t o = ms.op_begin; ms.kind == SK_REG && o < ms.op_begin + ms.op_count; o++)
+                if (!light_op(plan.ops[o].code)) pa.light = 0;
+        }
+        if (pa.has_wide) pa.light = 0;
         fill_direct(pa);
         return "";
+    }
+
+    // Ops of the light kernel variant (PassArgs::light): Euler-form Kronecker
+    // ops without a right diagonal, dense 1-qubit ops, phases, Hadamards,
+    // CNOTs.
+    static bool light_op(uint16_t code)
+    {
+        const int kind = code >> 8, var = code & 0xFF;
+        switch (kind) {
+        case OK_EKRON: return (var & 16) == 0;
+        case OK_DENSE: return __builtin_popcount(var) == 1;
+        case OK_PHASE:
+        case OK_H:
+        case OK_CNOT: return true;
+        default: return false;
+        }
     }
 
     // Fills the position lists of a pass whose tile holds the qubits of `set`.
@@ -1378,6 +1400,7 @@ std::string Plan::describe(bool verbose) const
         if (p.fuse_first || p.fuse_last)
             os << " direct=" << (p.fuse_first ? "first" : "") << (p.fuse_first && p.fuse_last ? "," : "")
                << (p.fuse_last ? "last" : "");
+        if (p.light && p.micro_count) os << " light";
         os << "\n";
         if (!verbose) continue;
         static const char* kinds[] = {"reg", "generic", "mfma", "mfma_bf16"};

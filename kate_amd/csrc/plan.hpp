@@ -93,7 +93,10 @@ struct PassArgs {
     // tile is not read from HBM but made in registers (amplitude 1 at state
     // offset synth_index, 0 elsewhere), which replaces the init pass.
     uint8_t synth_basis;
-    uint8_t pad_;
+    // Every op of the pass is in the light set (Euler-form Kronecker ops
+    // without a right diagonal, dense 1-qubit ops, phases, Hadamards,
+    // CNOTs): the pass runs on the kernel variant that dispatches only those.
+    uint8_t light;
     uint64_t first_g[16];
     uint64_t last_g[16];
     uint64_t synth_index;

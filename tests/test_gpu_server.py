@@ -78,3 +78,32 @@ def test_server_rejects_request_larger_than_budget(gpu):
         rid = srv.submit(K.Circuit(22, W.layered(22, 1, 0)), [1])
         with pytest.raises(K.KateError, match="does not fit"):
             srv.wait(rid)
+
+
+def test_server_28_qubit_request_alone_equals_mixed(gpu):
+    """A 28-qubit request (2 GiB complex64 state) served alone and amid
+    smaller requests gives the same <Z_S> bit for bit, and matches the same
+    circuit in complex128 on a simulator to FP32 accuracy."""
+    say("server 28q alone/mixed")
+    n = 28
+    gates = W.layered(n, 8, 77)
+    masks = [int(m) for m in W.zstring_masks(n, 8, 78)]
+    small = make_requests(10, 9, 12, 22)
+    budget = 6 << 30
+    with K.Server(dtype="c64", streams=4, max_active=16, hbm_budget_bytes=budget) as srv:
+        alone, st = srv.wait(srv.submit(K.Circuit(n, gates), masks))
+    assert st["nqubits"] == n
+    with K.Server(dtype="c64", streams=4, max_active=16, hbm_budget_bytes=budget) as srv:
+        ids = [srv.submit(K.Circuit(m_, g), mk, tp) for m_, g, tp, mk in small[:5]]
+        big = srv.submit(K.Circuit(n, gates), masks)
+        ids += [srv.submit(K.Circuit(m_, g), mk, tp) for m_, g, tp, mk in small[5:]]
+        mixed, _ = srv.wait(big)
+        rest = [srv.wait(i)[0] for i in ids]
+    assert np.array_equal(alone, mixed)
+    for (m_, g, tp, mk), z in zip(small, rest):
+        assert np.max(np.abs(z - oracle_z(m_, g, tp, mk))) < 2e-6
+    with K.Simulator(K.Circuit(n, gates), dtype="c128") as ref:
+        ref.execute()
+        z64 = ref.expect_z(masks)
+    say("28q FP32 server vs FP64 max |dZ|", float(np.max(np.abs(alone - z64))))
+    assert np.max(np.abs(alone - z64)) < 2e-6

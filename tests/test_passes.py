@@ -251,3 +251,16 @@ def test_fused_two_qubit_gates_run_as_their_factors():
     assert not any(d + " " in t_split or d + "\n" in t_split for d in dense2)
     assert any(d + " " in t_nosplit or d + "\n" in t_nosplit for d in dense2)
     assert p_nosplit >= p_split
+
+
+def test_light_passes_are_marked():
+    """Passes whose ops are all in the light set (Euler-form Kronecker ops
+    without a right diagonal, dense 1-qubit ops, phases, Hadamards, CNOTs)
+    run on the light kernel variant: every gate pass of the layered circuit,
+    none of the Haar brickwork (dense 2-qubit ops) or the QFT (CRk ops)."""
+    _, _, lay = K.Circuit(24, W.layered(24, 6, 1)).plan("c128")
+    gate = [ln for ln in lay.splitlines() if ln.startswith("pass") and " micro=0 " not in ln + " "]
+    assert gate and all(ln.endswith(" light") for ln in gate)
+    for gates in (W.brickwork(20, 6, 2), W.qft(16)):
+        _, _, t = K.Circuit(max(g for x in gates for g in x.qubits) + 1, gates).plan("c128")
+        assert " light" not in t
