@@ -14,6 +14,7 @@ on this host's cores on a bounded sample.
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -26,14 +27,16 @@ import kate_amd as K  # noqa: E402
 from kate_amd import workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per k_pass launch on the default workload (30 qubits, depth 20),
-# from rocprofv3 FETCH_SIZE and WRITE_SIZE runs, each scaled by what the same
-# counter reads on the gate-free passes of the same run (plain tile copies of
-# 16 GiB read + 16 GiB written: factors 2.000 and 1.000), averaged over the
-# 27 launches of the pass-by-pass run (every tile of every pass, as in the
-# roofline figures): 1.0004 x the algorithmic bytes
-# (profiles/k_pass_f64_30q_layered_hbm_counters.md).
-TRAFFIC_RATIO_30Q_LAYERED = 1.0004
+# HBM bytes per k_pass launch of the roofline set (the passes of the headline
+# execute() that read and write every amplitude), from rocprofv3 FETCH_SIZE
+# and WRITE_SIZE runs of `bench.py --passes-only execute`, each scaled by what
+# the same counter reads on the k_copy_tile launches of that run (16 GiB read
+# + 16 GiB written per launch: factors 2.000 and 1.000): 1.00003 x the
+# algorithmic bytes (profiles/k_pass_f64_30q_layered_hbm_counters.md).  The
+# ratio holds for the plan it was measured on; the bench reports it only when
+# the current plan of the default workload is that plan.
+TRAFFIC_RATIO_30Q_LAYERED = 1.00003
+TRAFFIC_PLAN_30Q_LAYERED = "nq_int=30 tile_bits=12 low=3 passes=27 (gate 27, remap 0) micro=151 ops=199"
 METRIC = "30q circuit ms + HBM % peak; served circuits/s + p99 at fidelity≥1-1e-6"
 
 
@@ -51,7 +54,6 @@ def plan_flops(circuit, dtype, n_int):
     one-qubit gates counts m dense 1-qubit gates); 8 per bit plus 8 per
     diagonal table for a Kronecker op in Euler form; sparse ops (H, CNOT,
     CRk, Toffoli, phases) count 0.  Returns a list, one entry per pass."""
-    import re
     os.environ["KATE_DESCRIBE_OPS"] = "1"
     try:
         text = circuit.plan(dtype)[2]
@@ -80,10 +82,30 @@ def plan_flops(circuit, dtype, n_int):
     return per_pass
 
 
+def pass_read_fraction(circuit, dtype):
+    """Per pass of an execute() from |0...0>: the fraction of the state its
+    loads read.  Pass 0 makes its tiles from the basis state and reads
+    nothing; a later pass skips the amplitudes on the state bits still known
+    to be zero (zero_in=k in the plan text: a fraction 2^-k).  Also returns
+    the plan's summary line."""
+    text = circuit.plan(dtype)[2]
+    frac = []
+    for ln in text.splitlines():
+        if ln.startswith("pass ") or ln.startswith("remap "):
+            m = re.search(r" zero_in=(\d+)", ln)
+            frac.append(0.0 if not frac else (0.5 ** int(m.group(1)) if m else 1.0))
+    return np.array(frac), text.splitlines()[0]
+
+
 def mfma_steps(circuit, dtype, mfma):
-    """Matrix-core micro-steps in the device plan of `circuit`."""
-    text = circuit.plan(dtype, mfma=mfma)[2]
-    return sum(ln.count(" mfma") for ln in text.splitlines())
+    """Matrix-core micro-steps in the device plan of `circuit` (the plan
+    text lists its steps by kind only with KATE_DESCRIBE_OPS set)."""
+    os.environ["KATE_DESCRIBE_OPS"] = "1"
+    try:
+        text = circuit.plan(dtype, mfma=mfma)[2]
+    finally:
+        os.environ.pop("KATE_DESCRIBE_OPS", None)
+    return sum(1 for ln in text.splitlines() if re.match(r"\s+step \d+ mfma", ln))
 
 
 def time_executes(sim, steps, warmup):
@@ -147,9 +169,11 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=256)
     ap.add_argument("--serve-qps", type=float, default=16.0)
     ap.add_argument("--serve-qps-high", type=float, default=1000.0)
-    ap.add_argument("--passes-only", choices=("execute", "dense"), default=None,
+    ap.add_argument("--passes-only", choices=("execute", "dense", "mfma"), default=None,
                     help="for profilers: one execute() of the headline circuit (execute), or one run of every "
-                         "pass over every tile (dense), after one untimed warm-up execute(); prints the pass times")
+                         "pass over every tile (dense), after one untimed warm-up execute(); mfma: one execute() "
+                         "of the 28-qubit layered circuit, complex64, GateMerger(5)-fused, on FP32 matrix-core "
+                         "windows; prints the pass times")
     ap.add_argument("--dtype", choices=("c128", "c64"), default="c128", help="state type of --passes-only")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -163,18 +187,36 @@ def main():
     n, depth = args.qubits, args.depth
     gates = W.layered(n, depth, 1)
     log(f"device: {K.device_info().splitlines()[0]}")
+    if args.passes_only == "mfma":
+        n, depth = 28, 20
+        c = K.Circuit(n, W.layered(n, depth, 2)).schedule().fuse(5)
+        with K.Simulator(c, dtype="c64", mfma=2) as sim:
+            sim.execute()
+            _, ms, tiles = sim.execute_timed()
+            sim.synchronize()
+        print(json.dumps({"mode": "mfma", "dtype": "c64", "qubits": n, "depth": depth,
+                          "mfma_steps": mfma_steps(c, "c64", 2), "passes": len(ms),
+                          "sum_ms": round(float(ms.sum()), 3), "pass_ms": [round(float(x), 3) for x in ms]}),
+              flush=True)
+        return
     if args.passes_only:
         with K.Simulator(K.Circuit(n, gates), dtype=args.dtype, support=args.passes_only == "execute") as sim:
             sim.execute()
+            copy_ms = None
             if args.passes_only == "execute":
                 _, ms, tiles = sim.execute_timed()
+                # the same passes' tiles copied in place (k_copy_tile): the
+                # HBM time each pass's access pattern allows
+                copy_ms = sim.copy_ceiling()
             else:
                 sim.init_basis(0)
                 ms, _ = sim.run_timed()
             sim.synchronize()
         print(json.dumps({"mode": args.passes_only, "dtype": args.dtype, "qubits": n, "depth": depth,
                           "passes": len(ms), "sum_ms": round(float(ms.sum()), 3),
-                          "pass_ms": [round(float(x), 3) for x in ms]}), flush=True)
+                          "pass_ms": [round(float(x), 3) for x in ms],
+                          "copy_ms": None if copy_ms is None else [round(float(x), 3) for x in copy_ms]}),
+              flush=True)
         return
 
     # ---- FP64 plain path (headline) ----
@@ -186,17 +228,24 @@ def main():
     ms64 = time_executes(sim, args.steps, args.warmup)
     t = sim.timings()
     # the same execute() with an event after every pass: the roofline of
-    # k_pass is quoted on its passes that launch every tile
+    # k_pass is quoted on its passes that read and write every amplitude
     _, pass_ms, tiles = sim.execute_timed()
     sim.synchronize()
-    bpp = 2.0 * (1 << max(n, 12)) * 16
-    full = tiles == tiles.max()
+    state_bytes = float(1 << max(n, 12)) * 16
+    bpp = 2.0 * state_bytes
+    read_frac, plan_line = pass_read_fraction(c, "c128")
+    if len(read_frac) != len(pass_ms):
+        raise RuntimeError(f"plan has {len(read_frac)} passes, execute_timed() {len(pass_ms)}")
+    full = (tiles == tiles.max()) & (read_frac == 1.0)
     mean_pass = float(pass_ms[full].mean())
     achieved = bpp / (mean_pass * 1e-3) / 1e9
-    circ_bytes = float(np.sum(bpp * tiles / tiles.max()))
+    # algorithmic bytes of the whole execute(): each pass reads its non-zero
+    # input and writes every tile it launches
+    circ_bytes = float(np.sum(state_bytes * (read_frac + tiles / tiles.max())))
     circuit_gbs = circ_bytes / (float(pass_ms.sum()) * 1e-3) / 1e9
-    log(f"fp64: {ms64:.2f} ms/circuit, {len(pass_ms)} passes ({int(full.sum())} over every tile), "
-        f"mean full pass {mean_pass:.3f} ms = {achieved:.0f} GB/s")
+    traffic = round(bpp * TRAFFIC_RATIO_30Q_LAYERED) if plan_line == TRAFFIC_PLAN_30Q_LAYERED else None
+    log(f"fp64: {ms64:.2f} ms/circuit, {len(pass_ms)} passes ({int(full.sum())} read and write every "
+        f"amplitude), mean of those {mean_pass:.3f} ms = {achieved:.0f} GB/s")
     # the HBM time each pass's access pattern allows: a plain in-place copy
     # of the same tiles (no arithmetic, no LDS), timed the same way
     ceil_ms = sim.copy_ceiling()
@@ -229,11 +278,11 @@ def main():
                    "depth": depth, "gates": len(gates), "state_gib": (1 << n) * 16 / 2 ** 30},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": round(bpp * TRAFFIC_RATIO_30Q_LAYERED) if (n, depth) == (30, 20) else None,
+                     "traffic": traffic,
                      "kernel": "k_pass<double,12>", "algorithmic_bytes_per_launch": bpp,
                      "mean_launch_ms": round(mean_pass, 4), "launches": int(full.sum()),
-                     "run": "the headline execute(): its passes that launch every tile (the others skip "
-                            "tiles that are still zero)",
+                     "run": "the headline execute(): its passes that read and write every amplitude (the "
+                            "first passes skip the tiles and amplitudes that are still zero)",
                      "copy_ceiling": {"mean_launch_ms": round(ceil_full, 4),
                                       "achieved": round(bpp / (ceil_full * 1e-3) / 1e9, 1),
                                       "frac_of_ceiling": round(ceil_full / mean_pass, 4),
@@ -333,9 +382,10 @@ def c2(steps=3):
 def c3(depth=24):
     """Haar brickwork, depth 24, long-range (q, q+16 mod n) pairs every 4th
     layer: 31 qubits complex128 and 32 qubits complex64 (32 GiB each), per-pass
-    HBM GB/s.  The same circuits compiled for a 3-qubit global block (Kate's
-    allocate(3) swap plan) show the in-HBM remap passes that replace its
-    multi-GPU qubit swaps."""
+    HBM GB/s; the complex64 circuit also on matrix-core windows (FP32 and
+    split-bf16 products).  The same circuits compiled for a 3-qubit global
+    block (Kate's allocate(3) swap plan) show the in-HBM remap passes that
+    replace its multi-GPU qubit swaps."""
     res = {}
     for n, dtype, amp in ((31, "c128", 16), (32, "c64", 8)):
         gates = W.brickwork(n, depth, 5)
@@ -344,6 +394,18 @@ def c3(depth=24):
         with K.Simulator(K.Circuit(n, gates), dtype=dtype) as sim:
             entry["ms"] = round(time_executes(sim, 2, 1), 3)
             entry["passes"] = pass_rates(sim, state)
+            if dtype == "c64":
+                # the same circuit on 5-qubit matrix-core windows (FP32
+                # products, and split-bf16 products), against the VALU state
+                for label, mfma in (("mfma_fp32", 2), ("mfma_bf16x3", 3)):
+                    c = K.Circuit(n, gates)
+                    with K.Simulator(c, dtype=dtype, mfma=mfma) as sm:
+                        ms = time_executes(sm, 2, 1)
+                        mx = sim.compare(sm)[3]
+                        entry[label] = {"ms": round(ms, 3), "speedup_vs_valu": round(entry["ms"] / ms, 3),
+                                        "mfma_steps": mfma_steps(c, dtype, mfma), "passes": sm.timings()["passes"],
+                                        "max_abs_err_vs_valu": mx,
+                                        "one_minus_fidelity_vs_valu": 1.0 - sim.fidelity(sm)}
         with K.Simulator(K.Circuit(n, gates).schedule().group(8).allocate(3), dtype=dtype) as sim:
             entry["swap_plan_ms"] = round(time_executes(sim, 1, 1), 3)
             entry["swap_plan_passes"] = pass_rates(sim, state)

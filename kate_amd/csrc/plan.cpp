@@ -1401,6 +1401,9 @@ std::string Plan::describe(bool verbose) const
             os << " direct=" << (p.fuse_first ? "first" : "") << (p.fuse_first && p.fuse_last ? "," : "")
                << (p.fuse_last ? "last" : "");
         if (p.light && p.micro_count) os << " light";
+        // state bits known zero at this pass's input in a run from |0...0>
+        // (its loads skip those amplitudes)
+        if (i < pass_zero.size() && pass_zero[i]) os << " zero_in=" << __builtin_popcountll(pass_zero[i]);
         os << "\n";
         if (!verbose) continue;
         static const char* kinds[] = {"reg", "generic", "mfma", "mfma_bf16"};

@@ -260,7 +260,22 @@ def test_light_passes_are_marked():
     none of the Haar brickwork (dense 2-qubit ops) or the QFT (CRk ops)."""
     _, _, lay = K.Circuit(24, W.layered(24, 6, 1)).plan("c128")
     gate = [ln for ln in lay.splitlines() if ln.startswith("pass") and " micro=0 " not in ln + " "]
-    assert gate and all(ln.endswith(" light") for ln in gate)
+    assert gate and all(" light" in ln.split(" zero_in=")[0][-6:] for ln in gate)
     for gates in (W.brickwork(20, 6, 2), W.qft(16)):
         _, _, t = K.Circuit(max(g for x in gates for g in x.qubits) + 1, gates).plan("c128")
         assert " light" not in t
+
+
+def test_plan_text_marks_zero_input_bits():
+    """A run from |0...0> reads, in pass i > 0, only the amplitudes whose
+    state bits outside the support are zero: the plan text's zero_in=k.  For
+    the layered circuit the first pass covers qubits 0..11 (so 30 - 12 bits
+    are still zero after it for a 30-qubit state), the count falls pass by
+    pass and the passes after the first full-read pass carry no mark."""
+    _, _, t = K.Circuit(30, W.layered(30, 20, 1)).plan("c128")
+    passes = [ln for ln in t.splitlines() if ln.startswith("pass")]
+    z = [int(ln.split(" zero_in=")[1]) if " zero_in=" in ln else 0 for ln in passes]
+    assert z[0] == 30 and z[1] == 30 - 12
+    first_full = z.index(0)
+    assert all(a > b for a, b in zip(z[:first_full], z[1:first_full + 1]))
+    assert all(v == 0 for v in z[first_full:])
