@@ -52,7 +52,8 @@ def plan_flops(circuit, dtype, n_int):
     tile, 2 per FMA-class instruction, per amplitude: 8 * 2^k for a dense
     k-qubit op (composed 1-qubit runs count once; a plain Kronecker op of m
     one-qubit gates counts m dense 1-qubit gates); 8 per bit plus 8 per
-    diagonal table for a Kronecker op in Euler form; sparse ops (H, CNOT,
+    diagonal table for a Kronecker op in Euler form, 4 per bit plus 8 in its
+    scaled form; sparse ops (H, CNOT,
     CRk, Toffoli, phases) count 0.  Returns a list, one entry per pass."""
     os.environ["KATE_DESCRIBE_OPS"] = "1"
     try:
@@ -68,10 +69,12 @@ def plan_flops(circuit, dtype, n_int):
             continue
         head = ln.split(" tcols=")[0]
         f = 0.0
-        for kind, code in re.findall(r" (dense|kron|ekron):(\d+)", head):
+        for kind, code in re.findall(r" (dense|kron|ekron|fkron):(\d+)", head):
             m = int(code)
             k = bin(m & 15).count("1")
-            if kind == "ekron":
+            if kind == "fkron":
+                f += 4.0 * (1 << n_int) * (k + 2)
+            elif kind == "ekron":
                 f += 8.0 * (1 << n_int) * (k + 1 + (m >> 4))
             else:
                 f += 8.0 * (1 << n_int) * (2 * k if kind == "kron" else (1 << k))

@@ -126,7 +126,14 @@ enum {
     /* plan the gates of a swap-free program only in the order given, instead
        of also in a canonical order of the same dependency graph (ASAP level,
        then lowest qubit) and keeping the cheaper plan */
-    KATE_PLAN_NO_REORDER = 32
+    KATE_PLAN_NO_REORDER = 32,
+    /* keep Euler-form Kronecker steps in the D1 * RX form instead of
+       scaling each rotation by its larger entry (one FMA per output component
+       instead of a multiply and an FMA; the scales go into the diagonal
+       table, and a rotation whose off-diagonal entry is the larger one is
+       applied with its two outputs exchanged, which the step's store undoes;
+       same result to rounding) */
+    KATE_PLAN_NO_FACTOR = 64
 };
 
 /* Wall-clock split of the last execute, as Kate's execute(true) prints it

@@ -41,13 +41,14 @@ class Options(C.Structure):
 
 
 PLAN_NO_EULER, PLAN_NO_DIRECT, PLAN_NO_SUPPORT, PLAN_NO_DEFER, PLAN_NO_SPLIT, PLAN_NO_REORDER = 1, 2, 4, 8, 16, 32
+PLAN_NO_FACTOR = 64
 
 
 def _plan_flags(euler: bool, direct: bool, support: bool = True, defer: bool = True, split: bool = True,
-                reorder: bool = True) -> int:
+                reorder: bool = True, factor: bool = True) -> int:
     return ((0 if euler else PLAN_NO_EULER) | (0 if direct else PLAN_NO_DIRECT) | (0 if support else PLAN_NO_SUPPORT)
             | (0 if defer else PLAN_NO_DEFER) | (0 if split else PLAN_NO_SPLIT)
-            | (0 if reorder else PLAN_NO_REORDER))
+            | (0 if reorder else PLAN_NO_REORDER) | (0 if factor else PLAN_NO_FACTOR))
 
 
 class Timings(C.Structure):
@@ -286,10 +287,10 @@ class Circuit:
 
     def plan(self, dtype: str = "c128", tile_qubits: int = 0, low_qubits: int = 0, mfma: int = 0,
              micro_bits: int = 0, euler: bool = True, direct: bool = True,
-             split: bool = True, reorder: bool = True) -> Tuple[int, int, str]:
+             split: bool = True, reorder: bool = True, factor: bool = True) -> Tuple[int, int, str]:
         """(gate passes, remap passes, text) of the device plan, without a GPU."""
         opt = Options({"c128": KATE_C128, "c64": KATE_C64}[dtype], tile_qubits, low_qubits, mfma, 0, 0, micro_bits,
-                      _plan_flags(euler, direct, split=split, reorder=reorder))
+                      _plan_flags(euler, direct, split=split, reorder=reorder, factor=factor))
         p, r = C.c_int32(), C.c_int32()
         buf = C.create_string_buffer(1 << 20)
         _check(lib().kate_circuit_plan(self._h, C.byref(opt), C.byref(p), C.byref(r), buf, 1 << 20), "plan")
@@ -311,19 +312,21 @@ class Simulator:
 
     def __init__(self, circuit: Circuit, dtype: str = "c128", tile_qubits: int = 0, low_qubits: int = 0,
                  device: int = 0, mfma: int = 0, micro_bits: int = 0, euler: bool = True, direct: bool = True,
-                 support: bool = True, defer: bool = False, split: bool = True, reorder: bool = True):
+                 support: bool = True, defer: bool = False, split: bool = True, reorder: bool = True,
+                 factor: bool = True):
         """`defer=True` lets execute() leave the trailing passes that only
         permute amplitudes (CNOT layers at the end of the circuit) unrun:
         its proba_state reads through them and any later read of the state
         runs them first.  Off by default, as Kate's execute() leaves the
         finished state in memory.  `split=False` applies dense 2-qubit gates
         of the form CNOT x (A (x) B) as 4x4 products instead of as their
-        factors; `reorder=False` plans the gates only in the order given."""
+        factors; `reorder=False` plans the gates only in the order given;
+        `factor=False` keeps Euler-form Kronecker steps unscaled."""
         self.n = circuit.n
         self.dtype = {"c128": KATE_C128, "c64": KATE_C64}[dtype]
         self.amp = np.complex128 if self.dtype == KATE_C128 else np.complex64
         opt = Options(self.dtype, tile_qubits, low_qubits, mfma, 0, device, micro_bits,
-                      _plan_flags(euler, direct, support, defer, split, reorder))
+                      _plan_flags(euler, direct, support, defer, split, reorder, factor))
         h = VP()
         _check(lib().kate_sim_create_from_circuit(circuit._h, C.byref(opt), C.byref(h)), "kate_sim_create")
         self._h = h
@@ -467,12 +470,12 @@ class Batch:
 
     def __init__(self, circuits: Sequence[Circuit], dtype: str = "c64", tile_qubits: int = 0, low_qubits: int = 0,
                  device: int = 0, mfma: int = 0, micro_bits: int = 0, euler: bool = True, direct: bool = True,
-                 wgs_per_cu: int = 0):
+                 wgs_per_cu: int = 0, factor: bool = True):
         self.circuits = list(circuits)
         self.nq = [c.n for c in self.circuits]
         arr = (VP * len(self.circuits))(*[c._h for c in self.circuits])
         opt = Options({"c128": KATE_C128, "c64": KATE_C64}[dtype], tile_qubits, low_qubits, mfma, wgs_per_cu,
-                      device, micro_bits, _plan_flags(euler, direct))
+                      device, micro_bits, _plan_flags(euler, direct, factor=factor))
         h = VP()
         _check(lib().kate_batch_create_from_circuits(arr, len(self.circuits), C.byref(opt), C.byref(h)),
                "kate_batch_create")

@@ -135,6 +135,9 @@ int kate_batch_create(const kate_program* programs, int32_t count, const kate_op
     kate::PlanOptions po;
     std::string err = kate::plan_options(opt, po);
     if (!err.empty()) return bfail(KATE_ERR_ARG, err);
+    // k_batch runs every pass on the full op set, which has no scaled
+    // Euler steps (the light kernel variants have them)
+    po.factor = false;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return bfail(KATE_ERR_NO_DEVICE, "no GPU visible");
 

@@ -37,17 +37,27 @@ struct alignas(2 * sizeof(R)) Cx {
     R re, im;
 };
 
+// Fused multiply-add written out: which product of an expression is fused
+// is then fixed by the source, not by how the compiler contracts each
+// inlined copy, so the direct and LDS paths of a step round identically.
+template <typename R>
+__device__ __forceinline__ R fmar(R a, R b, R c)
+{
+    if constexpr (sizeof(R) == 8) return __builtin_fma(a, b, c);
+    else return __builtin_fmaf(a, b, c);
+}
+
 template <typename R>
 __device__ __forceinline__ Cx<R> cmul(Cx<R> a, Cx<R> b)
 {
-    return Cx<R>{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+    return Cx<R>{fmar(a.re, b.re, -(a.im * b.im)), fmar(a.re, b.im, a.im * b.re)};
 }
 
 template <typename R>
 __device__ __forceinline__ Cx<R> cmac(Cx<R> acc, Cx<R> a, Cx<R> b)
 {
-    acc.re = acc.re + a.re * b.re - a.im * b.im;
-    acc.im = acc.im + a.re * b.im + a.im * b.re;
+    acc.re = fmar(-a.im, b.im, fmar(a.re, b.re, acc.re));
+    acc.im = fmar(a.im, b.re, fmar(a.re, b.im, acc.im));
     return acc;
 }
 
@@ -143,7 +153,7 @@ __device__ __forceinline__ void op_h(Cx<R> (&v)[N])
         const Cx<R> a = v[s], b = v[s | (1 << J)];
         const Cx<R> n0{(a.re + b.re) * inv, (a.im + b.im) * inv};
         v[s] = n0;
-        v[s | (1 << J)] = Cx<R>{n0.re - b.re * s2, n0.im - b.im * s2};
+        v[s | (1 << J)] = Cx<R>{fmar(-b.re, s2, n0.re), fmar(-b.im, s2, n0.im)};
     }
 }
 
@@ -233,13 +243,38 @@ __device__ __forceinline__ void op_ekron(Cx<R> (&v)[N], const Cx<R>* __restrict_
         for (int s = 0; s < N; s++) {
             if ((s >> b) & 1) continue;
             const Cx<R> a = v[s], c = v[s | (1 << b)];
-            v[s] = Cx<R>{x * a.re + y * c.im, x * a.im - y * c.re};
-            v[s | (1 << b)] = Cx<R>{x * c.re + y * a.im, x * c.im - y * a.re};
+            v[s] = Cx<R>{fmar(x, a.re, y * c.im), fmar(x, a.im, -(y * c.re))};
+            v[s | (1 << b)] = Cx<R>{fmar(x, c.re, y * a.im), fmar(x, c.im, -(y * a.re))};
         }
     }
     const Cx<R>* __restrict__ t1 = m + K;
 #pragma unroll
     for (int s = 0; s < N; s++) v[s] = cmul(t1[ext(MASK, s)], v[s]);
+}
+
+// OK_FKRON on every register bit: the rotations scaled to
+// [[1, -it], [-it, 1]] cost 2 FMAs per amplitude per bit (one per output
+// component), then one multiply by the table T (the scales, the D1
+// diagonal); coefficients: (t_b, 0) per bit in ascending order, then T.
+// The exchanged outputs of the flip mask are undone by the caller's store.
+template <typename R, int N>
+__device__ __forceinline__ void op_fkron(Cx<R> (&v)[N], const Cx<R>* __restrict__ m)
+{
+#pragma unroll
+    for (int b = 0; (1 << b) < N; b++) {
+        const R t = m[b].re;
+#pragma unroll
+        for (int s = 0; s < N; s++) {
+            if ((s >> b) & 1) continue;
+            const Cx<R> a = v[s], c = v[s | (1 << b)];
+            v[s] = Cx<R>{fmar(t, c.im, a.re), fmar(-t, c.re, a.im)};
+            v[s | (1 << b)] = Cx<R>{fmar(t, a.im, c.re), fmar(-t, a.re, c.im)};
+        }
+    }
+    constexpr int K = N == 16 ? 4 : (N == 8 ? 3 : 2);
+    const Cx<R>* __restrict__ t1 = m + K;
+#pragma unroll
+    for (int s = 0; s < N; s++) v[s] = cmul(t1[s], v[s]);
 }
 
 #define KATE_OPC(kind, var) (((kind) << 8) | (var))
@@ -587,20 +622,30 @@ __device__ __forceinline__ uint32_t reg_offset(int s, const uint32_t (&o)[4])
 // 32-bit words: a 16-bit field would be a vector load (there are no 16-bit
 // scalar loads), whose wait also drains every other vector memory operation
 // of the wave.
+// Returns the step's flip mask: the store writes register s where register
+// s ^ flip belongs (non-zero only for a lone OK_FKRON op).
 template <typename R, int N, bool LIGHT>
-__device__ __forceinline__ void reg_step_ops(Cx<R> (&v)[N], const MicroStep* __restrict__ msp, uint32_t op_begin,
-                                             uint32_t op_count, const MicroOp* __restrict__ ops,
-                                             const Cx<R>* __restrict__ coef)
+__device__ __forceinline__ uint32_t reg_step_ops(Cx<R> (&v)[N], const MicroStep* __restrict__ msp, uint32_t op_begin,
+                                                 uint32_t op_count, const MicroOp* __restrict__ ops,
+                                                 const Cx<R>* __restrict__ coef)
 {
     const uint32_t w0 = reinterpret_cast<const uint32_t*>(msp)[0];
     const uint32_t code0 = w0 >> 16, coef0 = reinterpret_cast<const uint32_t*>(msp)[3];
+    // the planner makes OK_FKRON only in light passes, as the lone op of a
+    // step, on every register bit
+    if constexpr (LIGHT) {
+        if (op_count == 1 && (code0 >> 8) == OK_FKRON) {
+            op_fkron<R, N>(v, coef + coef0);
+            return (code0 >> 4) & (N - 1);
+        }
+    }
     // Light complex128 passes: the common step (RX/RZ layers), one Kronecker
     // op in Euler form on every register bit, is taken before the op switch
     // (in the complex64 kernels the extra path would cost occupancy).
     if constexpr (LIGHT && sizeof(R) == 8) {
         if (op_count == 1 && code0 == KATE_OPC(OK_EKRON, N - 1)) {
             op_ekron<N - 1, false>(v, coef + coef0);
-            return;
+            return 0;
         }
     }
     for (uint32_t o = 0; o < op_count; o++) {
@@ -613,6 +658,7 @@ __device__ __forceinline__ void reg_step_ops(Cx<R> (&v)[N], const MicroStep* __r
         if constexpr (LIGHT) apply_op_light<R>(v, code, coef + cofs);
         else apply_op<R>(v, code, coef + cofs);
     }
+    return 0;
 }
 
 // Amplitude at state offset `off` of the basis state that the first pass
@@ -668,9 +714,10 @@ __device__ __forceinline__ void run_steps(const PassArgs& pa, Cx<R>* tile, const
             Cx<R> v[N];
 #pragma unroll
             for (int s = 0; s < N; s++) v[s] = lds_at(tile, b ^ reg_offset<NB>(s, oc));
-            reg_step_ops<R, N, LIGHT>(v, msp, op_begin, op_count, ops, coef);
+            const uint32_t f = reg_step_ops<R, N, LIGHT>(v, msp, op_begin, op_count, ops, coef);
+            const uint32_t bw = b ^ reg_offset<NB>((int)f, oc);
 #pragma unroll
-            for (int s = 0; s < N; s++) lds_at(tile, b ^ reg_offset<NB>(s, oc)) = v[s];
+            for (int s = 0; s < N; s++) lds_at(tile, bw ^ reg_offset<NB>(s, oc)) = v[s];
         } else if (WIDE) {
             if constexpr (sizeof(R) == 4 && NB == 4) {
                 Cx<float>* tf = reinterpret_cast<Cx<float>*>(tile);
@@ -760,7 +807,7 @@ __device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassA
 #pragma unroll
             for (int s = 0; s < N; s++) v[s] = state[gt ^ direct_reg_offset<NB>(pa.first_g, s)];
         }
-        reg_step_ops<R, N, LIGHT>(v, msp, op_begin, op_count, ops, coef);
+        const uint32_t f = reg_step_ops<R, N, LIGHT>(v, msp, op_begin, op_count, ops, coef);
         m_begin = 1;
         if (pa.micro_count == 1 && pa.fuse_last) {
             // the store of one thread can hit an amplitude that another
@@ -768,13 +815,15 @@ __device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassA
             // between threads): every load of the tile has completed (its
             // value was used above) once all threads pass this barrier
             __syncthreads();
-            const uint64_t gs = base ^ direct_thread_offset<TB, NB>(pa.last_g, tid);
+            const uint64_t gs = base ^ direct_thread_offset<TB, NB>(pa.last_g, tid) ^
+                                direct_reg_offset<NB>(pa.last_g, (int)f);
 #pragma unroll
             for (int s = 0; s < N; s++) state[gs ^ direct_reg_offset<NB>(pa.last_g, s)] = v[s];
             return;
         }
+        const uint32_t bw = b ^ reg_offset<NB>((int)f, oc);
 #pragma unroll
-        for (int s = 0; s < N; s++) lds_at(tile, b ^ reg_offset<NB>(s, oc)) = v[s];
+        for (int s = 0; s < N; s++) lds_at(tile, bw ^ reg_offset<NB>(s, oc)) = v[s];
     } else {   // load
         uint64_t tb = base;
 #pragma unroll
@@ -827,8 +876,9 @@ __device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassA
         Cx<R> v[N];
 #pragma unroll
         for (int s = 0; s < N; s++) v[s] = lds_at(tile, b ^ reg_offset<NB>(s, oc));
-        reg_step_ops<R, N, LIGHT>(v, msp, op_begin, op_count, ops, coef);
-        const uint64_t gs = base ^ direct_thread_offset<TB, NB>(pa.last_g, tid);
+        const uint32_t f = reg_step_ops<R, N, LIGHT>(v, msp, op_begin, op_count, ops, coef);
+        const uint64_t gs = base ^ direct_thread_offset<TB, NB>(pa.last_g, tid) ^
+                            direct_reg_offset<NB>(pa.last_g, (int)f);
 #pragma unroll
         for (int s = 0; s < N; s++) state[gs ^ direct_reg_offset<NB>(pa.last_g, s)] = v[s];
     } else {

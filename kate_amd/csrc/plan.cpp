@@ -779,6 +779,46 @@ struct Builder {
         return true;
     }
 
+    // A step's lone Euler-form op on every register bit, without a right
+    // diagonal, as OK_FKRON.  Each rotation is scaled by its larger entry:
+    // RX_b = s_b * [[1, -i t_b], [-i t_b, 1]] with s_b = x_b, t_b = y_b / x_b
+    // when |x_b| >= |y_b|, and RX_b = s_b * X_b * [[1, -i t_b], [-i t_b, 1]]
+    // with s_b = -i y_b, t_b = -x_b / y_b otherwise (X_b exchanges the two
+    // outputs of bit b).  With f the mask of exchanged bits,
+    // D1 * prod RX_b = X_f * T * prod [[1, -i t_b], [-i t_b, 1]] where
+    // T[e] = (prod s_b) * D1[e ^ f]: the kernel applies the scaled rotations
+    // and T, and the step's store writes register s where s ^ f belongs.
+    void factor_ekron(MicroOp& op, int nb)
+    {
+        using cd = std::complex<double>;
+        const int full = (1 << nb) - 1;
+        if ((op.code >> 8) != OK_EKRON || (op.code & 0xFF) != full) return;
+        double* c = plan.coef.data() + 2 * (size_t)op.coef;
+        cd scale = 1.0;
+        int flip = 0;
+        for (int i = 0; i < nb; i++) {
+            const double x = c[2 * i], y = c[2 * i + 1];
+            if (std::abs(x) >= std::abs(y)) {
+                c[2 * i] = y / x;
+                scale *= x;
+            } else {
+                c[2 * i] = -x / y;
+                scale *= cd(0.0, -y);
+                flip |= 1 << i;
+            }
+            c[2 * i + 1] = 0.0;
+        }
+        double* t = c + 2 * nb;
+        std::vector<cd> d1((size_t)1 << nb);
+        for (int e = 0; e <= full; e++) d1[(size_t)e] = cd(t[2 * e], t[2 * e + 1]);
+        for (int e = 0; e <= full; e++) {
+            const cd v = scale * d1[(size_t)(e ^ flip)];
+            t[2 * e] = v.real();
+            t[2 * e + 1] = v.imag();
+        }
+        op.code = (uint16_t)(OK_FKRON << 8 | full | flip << 4);
+    }
+
     // Emits the held gates of the micro bits in `mask`: a lone Hadamard keeps
     // Kate's Hadamard arithmetic, a diagonal becomes a phase op, the rest go
     // out together as one Kronecker op (in Euler form where that is cheaper).
@@ -1001,6 +1041,14 @@ struct Builder {
                 if (!light_op(plan.ops[o].code)) pa.light = 0;
         }
         if (pa.has_wide) pa.light = 0;
+        // the light kernels take a step's lone Euler-form op on every
+        // register bit in the scaled form
+        for (uint32_t m = pa.micro_begin; opt.factor && pa.light && m < plan.micro.size(); m++) {
+            MicroStep& ms = plan.micro[m];
+            if (ms.op_count != 1) continue;
+            factor_ekron(plan.ops[ms.op_begin], ms.nbits);
+            ms.op0_code = plan.ops[ms.op_begin].code;
+        }
         fill_direct(pa);
         return "";
     }
@@ -1013,6 +1061,7 @@ struct Builder {
         const int kind = code >> 8, var = code & 0xFF;
         switch (kind) {
         case OK_EKRON: return (var & 16) == 0;
+        case OK_FKRON: return true;
         case OK_DENSE: return __builtin_popcount(var) == 1;
         case OK_PHASE:
         case OK_H:
@@ -1407,7 +1456,7 @@ std::string Plan::describe(bool verbose) const
         os << "\n";
         if (!verbose) continue;
         static const char* kinds[] = {"reg", "generic", "mfma", "mfma_bf16"};
-        static const char* okinds[] = {"dense", "phase", "h", "cnot", "crk", "toff", "kron", "ekron"};
+        static const char* okinds[] = {"dense", "phase", "h", "cnot", "crk", "toff", "kron", "ekron", "fkron"};
         for (uint32_t m = 0; m < p.micro_count; m++) {
             const MicroStep& ms = micro[p.micro_begin + m];
             os << "    step " << m << " " << kinds[ms.kind] << " bits=[";

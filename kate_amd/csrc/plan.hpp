@@ -20,6 +20,9 @@ enum OpKind : uint16_t {
     OK_KRON = 6,   // variant = mask of micro bits; one 2x2 per set bit, ascending, applied in turn
     OK_EKRON = 7,  // variant = mask of micro bits (2+ bits) | 16 if the right diagonal is present;
                    // the same product with every 2x2 in Euler form D1 * RX * D2
+    OK_FKRON = 8,  // variant = mask of micro bits | flip mask << 4: OK_EKRON without D2 on every
+                   // register bit, each RX scaled to [[1, -it], [-it, 1]] (|t| <= 1; scales in the
+                   // D1 table); the step's store writes register s to the place of s ^ flip
 };
 
 enum StepKind : uint8_t {
@@ -138,6 +141,7 @@ struct PlanOptions {
     int micro_bits = 4;      // register micro-steps span 3 or 4 tile bits (8 or 16 amplitudes per thread)
     int amp_bytes = 16;      // 16 (complex128) or 8 (complex64): LDS byte offsets in the plan
     bool euler = true;       // Kronecker ops of 2+ composed 1-qubit gates in Euler form (OK_EKRON)
+    bool factor = true;      // a step's lone Euler-form op on every register bit scaled to OK_FKRON
     bool direct = true;      // first / last SK_REG step of a pass loads / stores HBM directly
     bool auto_low = false;   // build the plan for several low_qubits and keep the cheapest
     bool support = true;     // runs from |0...0> skip tiles known to be zero (used by the simulator)

@@ -350,12 +350,13 @@ std::string plan_options(const kate_options& opt, kate::PlanOptions& po)
     po.amp_bytes = opt.dtype == KATE_C128 ? 16 : 8;
     if (opt.plan_flags &
         ~(KATE_PLAN_NO_EULER | KATE_PLAN_NO_DIRECT | KATE_PLAN_NO_SUPPORT | KATE_PLAN_NO_DEFER | KATE_PLAN_NO_SPLIT |
-          KATE_PLAN_NO_REORDER))
+          KATE_PLAN_NO_REORDER | KATE_PLAN_NO_FACTOR))
         return "unknown plan_flags bits";
     po.split = !(opt.plan_flags & KATE_PLAN_NO_SPLIT);
     po.reorder = !(opt.plan_flags & KATE_PLAN_NO_REORDER);
     po.defer = !(opt.plan_flags & KATE_PLAN_NO_DEFER);
     po.euler = !(opt.plan_flags & KATE_PLAN_NO_EULER);
+    po.factor = !(opt.plan_flags & KATE_PLAN_NO_FACTOR);
     po.direct = !(opt.plan_flags & KATE_PLAN_NO_DIRECT);
     po.support = !(opt.plan_flags & KATE_PLAN_NO_SUPPORT);
     if (po.micro_bits != 3 && po.micro_bits != 4) return "micro_bits must be 3 or 4";

@@ -2,6 +2,7 @@
 outputs.  FP64 (complex128) must agree with Kate's CPU simulator within
 1e-12 max-abs amplitude error; FP32 (complex64) within 1 - fidelity <= 1e-6."""
 import os
+import re
 import subprocess
 import sys
 
@@ -288,3 +289,53 @@ def test_overlap_fidelity_on_device(gpu):
         assert abs(ov - np.vdot(sa, sb)) < 1e-10
         assert abs(na - np.vdot(sa, sa).real) < 1e-12 and abs(nb - np.vdot(sb, sb).real) < 1e-10
         assert 1 - a.fidelity(b) <= FID32
+
+
+@pytest.mark.parametrize("n,depth,seed,dtype,mb,direct,tile", [
+    (18, 6, 11, "c128", 4, True, 12), (18, 6, 11, "c128", 4, False, 12), (16, 5, 12, "c128", 3, True, 12),
+    (20, 8, 13, "c128", 4, True, 10), (20, 8, 13, "c64", 4, True, 12), (17, 6, 14, "c64", 4, False, 11)])
+def test_factored_euler_steps_match_oracle(gpu, n, depth, seed, dtype, mb, direct, tile):
+    """Light passes apply a step's lone Euler-form op in the scaled form
+    (fkron), whose stores undo the exchanged outputs of its flip mask: the
+    state matches the oracle (complex128 within 1e-12, complex64 within
+    1 - F <= 1e-6) and the unscaled plan to rounding, through LDS stores,
+    direct stores and 3-bit micro-steps."""
+    say("factored", n, depth, seed, dtype, mb, direct, tile)
+    gates = W.layered(n, depth, seed)
+    c = K.Circuit(n, gates)
+    _, _, text = c.plan(dtype, tile_qubits=tile, micro_bits=mb, direct=direct)
+    assert "light" in text
+    opt = dict(tile_qubits=tile, micro_bits=mb, direct=direct)
+    st, _ = gpu_state(c, dtype, **opt)
+    st0, _ = gpu_state(c, dtype, factor=False, **opt)
+    ref = O.simulate(n, gates)
+    if dtype == "c128":
+        assert np.max(np.abs(st - ref)) <= TOL64
+        assert np.max(np.abs(st - st0)) <= 1e-13
+    else:
+        assert 1.0 - fidelity(st, ref) <= FID32
+        assert 1.0 - fidelity(st, st0) <= FID32
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_factored_single_step_direct_pass(gpu, dtype, monkeypatch):
+    """A pass of one micro-step, loaded from HBM into registers and stored
+    straight back (direct first and last step), whose lone op is a scaled
+    Euler-form op with a non-zero flip mask: from |0...0> and from a product
+    state, against the oracle."""
+    n = 13
+    gates = [g for g in W.layered(n, 1, 1) if len(g.qubits) == 1 and g.qubits[0] >= 9]
+    monkeypatch.setenv("KATE_DESCRIBE_OPS", "1")
+    _, _, text = K.Circuit(n, gates).plan(dtype)
+    monkeypatch.delenv("KATE_DESCRIBE_OPS")
+    m = [int(x) for x in re.findall(r" fkron:(\d+)", text)]
+    assert "micro=1 " in text and "direct=first,last" in text and m and m[0] >> 4
+    rng = np.random.default_rng(5)
+    tp = np.column_stack([rng.uniform(0, 1, n), rng.uniform(0, 6, n)])
+    for proba_in in (None, tp):
+        st, _ = gpu_state(K.Circuit(n, gates), dtype, proba_in=proba_in)
+        ref = O.simulate(n, gates, proba_in)
+        if dtype == "c128":
+            assert np.max(np.abs(st - ref)) <= TOL64
+        else:
+            assert 1.0 - fidelity(st, ref) <= FID32

@@ -146,9 +146,9 @@ def test_plan_three_bit_micro_steps(seed, monkeypatch):
         # every op addresses micro bits 0..2 only
         for kind, code in re.findall(r" (\w+):(\d+)", ln.split(" tcols=")[0]):
             c = int(code)
-            idx = {"dense": [], "kron": [], "ekron": [], "h": [c], "phase": [c], "cnot": [c // 4, c % 4],
-                   "crk": [c // 4, c % 4], "toff": [c // 16, (c // 4) % 4, c % 4]}[kind]
-            mask = c & 15 if kind == "ekron" else c
+            idx = {"dense": [], "kron": [], "ekron": [], "fkron": [], "h": [c], "phase": [c],
+                   "cnot": [c // 4, c % 4], "crk": [c // 4, c % 4], "toff": [c // 16, (c // 4) % 4, c % 4]}[kind]
+            mask = c & 15 if kind in ("ekron", "fkron") else c
             assert all(i < 3 for i in idx) and (kind not in ("dense", "kron", "ekron") or mask < 8), ln
     if any(len(g.qubits) == 4 and g.id == W.DENSE for g in gates):
         assert " generic bits=[" in text
@@ -174,7 +174,7 @@ def test_plan_euler_form_ops(monkeypatch):
         _, _, text = K.Circuit(n, gates).plan("c128", **kw)
         return re.findall(r" (kron|ekron):(\d+)", text)
 
-    lay = kinds(W.layered(n, 4, 1))
+    lay = kinds(W.layered(n, 4, 1), factor=False)
     assert lay and all(k == "ekron" and int(c) < 16 and bin(int(c)).count("1") >= 2 for k, c in lay)
     plain = kinds(W.layered(n, 4, 1), euler=False)
     assert plain and all(k == "kron" for k, _ in plain)
@@ -183,6 +183,30 @@ def test_plan_euler_form_ops(monkeypatch):
     assert any(k == "ekron" and int(c) >= 16 for k, c in haar)
     # with the right diagonal, a 2-bit Euler op is no cheaper than the plain one
     assert all(k == "kron" or int(c) < 16 or bin(int(c) & 15).count("1") >= 3 for k, c in haar)
+
+
+def test_plan_factored_euler_ops(monkeypatch):
+    """In light passes a step whose lone op is an Euler-form Kronecker op on
+    all four register bits is planned in the scaled form (fkron, variant =
+    15 | flip mask << 4); steps with other ops keep ekron; factor=False
+    keeps every ekron.  Random RX angles make both flip values common."""
+    monkeypatch.setenv("KATE_DESCRIBE_OPS", "1")
+    n = 20
+    _, _, text = K.Circuit(n, W.layered(n, 8, 3)).plan("c128")
+    steps = [ln for ln in text.splitlines() if ln.lstrip().startswith("step ")]
+    fk = [int(c) for ln in steps for c in re.findall(r" fkron:(\d+)", ln)]
+    assert fk and all(c & 15 == 15 for c in fk)
+    assert len({c >> 4 for c in fk}) > 4
+    for ln in steps:
+        if " fkron:" in ln:
+            ops = re.findall(r" (\w+):(\d+)", ln.split(" tcols=")[0])
+            assert len(ops) == 1, ln
+    _, _, plain = K.Circuit(n, W.layered(n, 8, 3)).plan("c128", factor=False)
+    assert " fkron:" not in plain
+    assert plain.count(" ekron:15") == text.count(" fkron:") + text.count(" ekron:15")
+    # passes with right diagonals (Haar layers) use the full kernel: no fkron
+    _, _, haar = K.Circuit(n, W.haar_layers(n, 4, 2)).plan("c128")
+    assert " fkron:" not in haar
 
 
 def test_direct_plans_take_the_direct_path():
