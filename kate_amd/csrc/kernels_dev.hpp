@@ -886,16 +886,25 @@ __device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassA
     }
 }
 
+// Waves per SIMD that the LDS tiles of one CU (160 KiB) allow.
+template <typename R, int TB, int NB>
+constexpr int kLdsWavesPerSimd =
+    (163840 / ((1 << TB) * 2 * (int)sizeof(R))) * ((1 << (TB - NB)) / 64) / 4;
+
 // Minimum waves per SIMD the register allocation must allow: two for
 // complex128 16-amplitude threads (their 64 KiB LDS tiles allow two per CU),
-// four for the other light variants (as many as their LDS tiles allow);
-// the compiler's own choice otherwise.
-template <typename R, int NB, bool LIGHT>
-constexpr int kPassWavesPerEU = (sizeof(R) == 8 && NB == 4) ? 2 : (LIGHT ? 4 : 1);
+// for the other light variants as many as their LDS tiles allow, at most
+// four; the compiler's own choice otherwise.
+template <typename R, int TB, int NB, bool LIGHT>
+constexpr int kPassWavesPerEU =
+    (sizeof(R) == 8 && NB == 4) ? 2
+                                : (LIGHT ? (kLdsWavesPerSimd<R, TB, NB> < 1 ? 1
+                                            : (kLdsWavesPerSimd<R, TB, NB> > 4 ? 4 : kLdsWavesPerSimd<R, TB, NB>))
+                                         : 1);
 
 // LIGHT: the pass's ops are all in the light set (apply_op_light).
 template <typename R, int TB, bool WIDE, int NB, bool LIGHT>
-__global__ __launch_bounds__(1 << (TB - NB)) __attribute__((amdgpu_waves_per_eu(kPassWavesPerEU<R, NB, LIGHT>)))
+__global__ __launch_bounds__(1 << (TB - NB)) __attribute__((amdgpu_waves_per_eu(kPassWavesPerEU<R, TB, NB, LIGHT>)))
 void k_pass(Cx<R>* __restrict__ state, PassArgs pa,
                                                          const MicroStep* __restrict__ micro,
                                                          const MicroOp* __restrict__ ops,
