@@ -83,7 +83,8 @@ typedef struct kate_program {
 /* Execution options (no counterpart in Kate, whose kernels have one shape). */
 typedef struct kate_options {
     int32_t dtype;        /* kate_dtype; KATE_C128 is the plain path */
-    int32_t tile_qubits;  /* 10..13; 0 = 12 for KATE_C128, 13 for KATE_C64 */
+    int32_t tile_qubits;  /* 10..13; 0 = 12 for KATE_C128; for KATE_C64 the
+                             planner takes 13 or 12 per circuit */
     int32_t low_qubits;   /* qubits always in a tile (coalesced run = 2^low amplitudes); 0 = the
                              planner tries 2-5 (complex128) / 3-6 (complex64) and keeps the plan
                              with the lowest estimated HBM + register-step cost */

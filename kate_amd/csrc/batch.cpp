@@ -136,8 +136,10 @@ int kate_batch_create(const kate_program* programs, int32_t count, const kate_op
     std::string err = kate::plan_options(opt, po);
     if (!err.empty()) return bfail(KATE_ERR_ARG, err);
     // k_batch runs every pass on the full op set, which has no scaled
-    // Euler steps (the light kernel variants have them)
+    // Euler steps (the light kernel variants have them), with one tile size
+    // for every circuit of the batch
     po.factor = false;
+    po.auto_tile = false;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return bfail(KATE_ERR_NO_DEVICE, "no GPU visible");
 

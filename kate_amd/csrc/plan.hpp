@@ -144,6 +144,7 @@ struct PlanOptions {
     bool factor = true;      // a step's lone Euler-form op on every register bit scaled to OK_FKRON
     bool direct = true;      // first / last SK_REG step of a pass loads / stores HBM directly
     bool auto_low = false;   // build the plan for several low_qubits and keep the cheapest
+    bool auto_tile = false;  // complex64: 13-qubit tiles for all-light plans, else 12 (tile_bits ignored)
     bool support = true;     // runs from |0...0> skip tiles known to be zero (used by the simulator)
     bool defer = true;       // execute() measures through the trailing permutation passes (simulator)
     bool split = true;       // dense 2-qubit gates that are CNOT x (A (x) B) products run as their factors

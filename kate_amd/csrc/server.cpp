@@ -94,7 +94,8 @@ struct Request {
     kate::Program prog;
     kate::Plan plan;
     std::vector<uint64_t> masks;  // virtual-position masks, [0] = 0 (norm)
-    std::vector<double> product;  // factors (complex128) or empty
+    std::vector<double> theta_phi;  // product-state input (2 per qubit) or empty
+    std::vector<double> product;    // its factors (complex128) over the plan's qubits, set when planned
     std::string error;
     bool planned = false;
     // arena ranges
@@ -111,6 +112,26 @@ struct Request {
     double t_submit = 0, t_admit = 0, t_done = 0;
     int state = 0;  // 0 queued, 1 running, 2 done, 3 failed
 };
+
+// Kate's product-state input (theta, phi per qubit; simulator.hpp's
+// initialize(proba_state)) as one (|0>, |1>) factor pair per state qubit of
+// the plan, qubits the program does not have left at |0>.
+static void product_factors(Request& r)
+{
+    const int n = r.prog.nqubits, m = r.plan.nq_int;
+    r.product.assign(4 * (size_t)m, 0.0);
+    for (int j = 0; j < m; j++) r.product[2 * (size_t)j] = 1.0;
+    const double pi = 3.1415926535897932384626433;
+    for (int i = 0; i < n; i++) {
+        const double th = r.theta_phi[2 * (size_t)i], ph = r.theta_phi[2 * (size_t)i + 1];
+        const int v = r.prog.initial_permutation[(size_t)i];
+        const double sn = std::sin(th * pi / 2);
+        r.product[2 * (size_t)v] = std::cos(th * pi / 2);
+        r.product[2 * (size_t)v + 1] = 0.0;
+        r.product[2 * ((size_t)v + m)] = std::cos(ph) * sn;
+        r.product[2 * ((size_t)v + m) + 1] = std::sin(ph) * sn;
+    }
+}
 
 }  // namespace
 
@@ -246,6 +267,7 @@ void worker_loop(kate_server* s)
             for (auto& r : unplanned) {
                 std::string err = kate::build_plan(r->prog, s->po, r->plan);
                 if (err.empty() && s->opt.dtype == KATE_C64) r->coef32.assign(r->plan.coef.begin(), r->plan.coef.end());
+                if (err.empty() && !r->theta_phi.empty()) product_factors(*r);
                 if (!err.empty()) r->error = "planning: " + err;
             }
             lk.lock();
@@ -358,6 +380,15 @@ int kate_server_create(const kate_server_options* options, kate_server** out)
     kate::PlanOptions po;
     std::string err = kate::plan_options(ko, po);
     if (!err.empty()) return sfail(KATE_ERR_ARG, err);
+    if (po.auto_tile) {
+        // one tile size, planned once per request on the scheduler thread:
+        // 12-qubit tiles (a mix of 128 circuits of 12..30 qubits at
+        // saturation: 202 circuits/s against 181 with 13-qubit tiles and 143
+        // with both planned per request)
+        po.auto_tile = false;
+        po.tile_bits = 12;
+        po.min_qubits = 12;
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return sfail(KATE_ERR_NO_DEVICE, "no GPU visible");
     if (o.hbm_budget_bytes <= 0) {
@@ -413,21 +444,7 @@ int kate_server_submit(kate_server* s, const kate_program* program, const double
             if ((zmasks[j] >> r->prog.final_inverse_permutation[(size_t)p]) & 1ull) v |= 1ull << p;
         r->masks.push_back(v);
     }
-    if (theta_phi) {
-        const int m = std::max(n, s->po.min_qubits);
-        r->product.assign(4 * (size_t)m, 0.0);
-        for (int j = 0; j < m; j++) r->product[2 * (size_t)j] = 1.0;
-        const double pi = 3.1415926535897932384626433;
-        for (int i = 0; i < n; i++) {
-            const double th = theta_phi[2 * i], ph = theta_phi[2 * i + 1];
-            const int v = r->prog.initial_permutation[(size_t)i];
-            const double sn = std::sin(th * pi / 2);
-            r->product[2 * (size_t)v] = std::cos(th * pi / 2);
-            r->product[2 * (size_t)v + 1] = 0.0;
-            r->product[2 * ((size_t)v + m)] = std::cos(ph) * sn;
-            r->product[2 * ((size_t)v + m) + 1] = std::sin(ph) * sn;
-        }
-    }
+    if (theta_phi) r->theta_phi.assign(theta_phi, theta_phi + 2 * (size_t)n);
     {
         std::lock_guard<std::mutex> g(s->mu);
         r->id = s->next_id++;

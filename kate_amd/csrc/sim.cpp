@@ -336,10 +336,10 @@ namespace kate {
 std::string plan_options(const kate_options& opt, kate::PlanOptions& po)
 {
     if (opt.dtype != KATE_C128 && opt.dtype != KATE_C64) return "dtype must be KATE_C128 or KATE_C64";
-    // 64 KiB LDS tiles by default: 4096 complex128 or 8192 complex64
-    // amplitudes (two workgroups per CU either way; the complex64 tile's
-    // extra qubit lets 256 B runs keep 8 free tile qubits)
+    // complex128: 12-qubit tiles (64 KiB of LDS) by default; complex64: the
+    // planner picks 13 (64 KiB) or 12 (32 KiB) per circuit
     po.tile_bits = opt.tile_qubits ? opt.tile_qubits : (opt.dtype == KATE_C128 ? 12 : 13);
+    po.auto_tile = opt.tile_qubits == 0 && opt.dtype == KATE_C64;
     po.low_qubits = opt.low_qubits ? opt.low_qubits : (opt.dtype == KATE_C128 ? 3 : 4);
     po.auto_low = opt.low_qubits == 0;  // 0: the planner picks it per circuit
     // the reductions read whole 4096-amplitude blocks
