@@ -339,3 +339,60 @@ def test_factored_single_step_direct_pass(gpu, dtype, monkeypatch):
             assert np.max(np.abs(st - ref)) <= TOL64
         else:
             assert 1.0 - fidelity(st, ref) <= FID32
+
+
+def test_factored_steps_special_angles(gpu):
+    """Scaled Euler steps at the edges of their case split: RX angles whose
+    cos and sin halves tie (pi/2), vanish (0, pi: the Euler form then falls
+    back to the plain product) or nearly vanish (1e-9, pi - 1e-9, so that
+    t = y/x or x/y is tiny), beside random ones, in the layered structure;
+    against the oracle and the unscaled plan."""
+    say("factored special angles")
+    n = 16
+    rng = np.random.default_rng(31)
+    special = [0.0, np.pi / 2, np.pi, np.pi - 1e-9, 1e-9, 3 * np.pi / 2, 2 * np.pi - 1e-7, -np.pi / 2]
+    gates = []
+    for layer in range(8):
+        for q in range(n):
+            th = special[(q + layer) % len(special)] if (q + layer) % 3 else float(rng.uniform(0, 2 * np.pi))
+            gates.append(W.Gate(W.DENSE, (q,), 0, W.rx(th)))
+            gates.append(W.Gate(W.DENSE, (q,), 0, W.rz(float(rng.uniform(0, 2 * np.pi)))))
+        for i in range(n - 1):
+            gates.append(W.Gate(W.CNOT, (i + 1, i)))
+    ref = O.simulate(n, gates)
+    for dtype in ("c128", "c64"):
+        st, _ = gpu_state(K.Circuit(n, gates), dtype)
+        st0, _ = gpu_state(K.Circuit(n, gates), dtype, factor=False)
+        if dtype == "c128":
+            assert np.max(np.abs(st - ref)) <= TOL64
+            assert np.max(np.abs(st - st0)) <= 1e-13
+        else:
+            assert 1.0 - fidelity(st, ref) <= FID32
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_factored_steps_in_swap_programs(gpu, dtype):
+    """A layered circuit compiled for 4 GPUs: its gate passes run the scaled
+    Euler steps on the light kernel, its swaps run as remap passes; against
+    the oracle run of the compiled program and, in real qubit order, against
+    the swap-free circuit."""
+    say("factored swap program", dtype)
+    n = 17
+    gates = W.layered(n, 6, 77)
+    c = K.Circuit(n, gates).schedule().group(8).allocate(2)
+    assert any(k == 0 for k, _ in c.instructions())
+    _, remaps, text = c.plan(dtype)
+    assert remaps > 0 and " light" in text
+    prog = compiled_from_circuit(c)
+    st, _ = gpu_state(c, dtype)
+    ref = O.run(prog)
+    if dtype == "c128":
+        assert np.max(np.abs(st - ref)) <= TOL64
+    else:
+        assert 1.0 - fidelity(st, ref) <= FID32
+    st_real, _ = gpu_state(c, dtype, real_order=True)
+    st0, _ = gpu_state(K.Circuit(n, gates), dtype)
+    if dtype == "c128":
+        assert np.max(np.abs(st_real - st0)) <= 1e-12
+    else:
+        assert 1.0 - fidelity(st_real, st0) <= FID32
