@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # ratio holds for the plan it was measured on; the bench reports it only when
 # the current plan of the default workload is that plan.
 TRAFFIC_RATIO_30Q_LAYERED = 1.00001
-TRAFFIC_PLAN_30Q_LAYERED = "nq_int=30 tile_bits=12 low=3 passes=27 (gate 27, remap 0) micro=151 ops=199"
+TRAFFIC_PLAN_30Q_LAYERED = "nq_int=30 tile_bits=12 low=3 passes=25 (gate 25, remap 0) micro=151 ops=199"
 METRIC = "30q circuit ms + HBM % peak; served circuits/s + p99 at fidelity≥1-1e-6"
 
 

@@ -134,7 +134,13 @@ enum {
        table, and a rotation whose off-diagonal entry is the larger one is
        applied with its two outputs exchanged, which the step's store undoes;
        same result to rounding) */
-    KATE_PLAN_NO_FACTOR = 64
+    KATE_PLAN_NO_FACTOR = 64,
+    /* run two or more trailing permutation passes (the CNOT layers at the
+       end of a circuit) as they are planned instead of as one out-of-place
+       pass into a second state buffer (which is allocated only when the
+       GPU has room for both states and 1 GiB more; results bit for bit the
+       same) */
+    KATE_PLAN_NO_SCATTER = 128
 };
 
 /* Wall-clock split of the last execute, as Kate's execute(true) prints it
@@ -216,7 +222,10 @@ int kate_sim_read_state(kate_sim* sim, void* host, uint64_t bytes, int32_t real_
 int kate_sim_overlap(kate_sim* a, kate_sim* b, double* out5);
 /* No counterpart in Kate: overwrite the state (virtual order). */
 int kate_sim_write_state(kate_sim* sim, const void* host, uint64_t bytes);
-/* Device address of the state (Simulator::gpu_qbits_states[0]). */
+/* Device address of the state (Simulator::gpu_qbits_states[0]).  A plan
+ * whose trailing permutation passes run as one out-of-place pass (see
+ * KATE_PLAN_NO_SCATTER) moves the state between two buffers on every
+ * execute(): call this again after each execute() before using the address. */
 int kate_sim_device_state(kate_sim* sim, void** dptr);
 int kate_sim_last_timings(kate_sim* sim, kate_timings* out);
 int kate_sim_synchronize(kate_sim* sim);

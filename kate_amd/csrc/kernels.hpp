@@ -61,6 +61,10 @@ int launch_pass(int dtype, int tile_bits, void* state, int nq_int, const PassArg
 // Calibration: every tile of the pass read and written back in place
 // (load positions, no arithmetic, no LDS): the HBM time of its access pattern.
 int launch_copy_tiles(int dtype, int tile_bits, void* state, int nq_int, const PassArgs& pa, void* stream);
+// The out-of-place permutation pass (Plan::scatter_pass): every tile of `src`
+// to runs of `dst`.
+int launch_scatter(int dtype, int tile_bits, const void* src, void* dst, const PassArgs& pa, const ScatterArgs& sa,
+                   void* stream);
 // |index> (basis state).
 int launch_init_basis(int dtype, void* state, int nq_int, uint64_t index, void* stream);
 // Product state: factor[b*nq + j] (complex128) for bit value b of qubit j.

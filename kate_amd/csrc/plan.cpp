@@ -1281,6 +1281,93 @@ static void plan_trailing(Plan& plan)
     }
 }
 
+// Two or more trailing permutation passes (together the map A, known as
+// trail_inv = A^-1) as one out-of-place pass.  Its tile holds the low
+// qubits and every qubit that A^-1 of a low-qubit offset involves, so that
+// each output run of 2^low amplitudes has all its sources in one tile; the
+// tile is read in runs as usual and written through A: the store index has
+// the low offset bits first, then tile-bit images completing a basis of
+// their span, and each store index bit k reads LDS through store_cols[k]
+// (its combination of tile bits) and writes to out_tile[k].
+static void make_scatter(Plan& plan, int amp_bytes)
+{
+    const size_t b = plan.trail_begin, np = plan.passes.size();
+    if (np < b + 2 || plan.trail_inv.empty() || plan.remap_passes) return;
+    for (size_t k = b; k < np; k++)
+        if (plan.pass_is_remap[k]) return;
+    const int n = plan.nq_int, t = plan.tile_bits, low = std::min(plan.low_qubits, plan.tile_bits);
+    if (n > 64) return;
+    std::vector<uint64_t> A;
+    if (!gf2_inverse(plan.trail_inv, A)) return;
+    uint64_t set = (1ull << low) - 1ull;
+    for (int j = 0; j < low; j++) set |= plan.trail_inv[(size_t)j];
+    if (popc(set) > t) return;
+    set = pad_set(set, t, n);
+    PassArgs pa{};
+    int tb = 0, rb = 0;
+    for (int q = 0; q < n; q++) {
+        if ((set >> q) & 1ull) pa.load_pos[tb++] = (uint8_t)q;
+        else pa.rest_pos[rb++] = (uint8_t)q;
+    }
+    pa.tile_bits = (uint8_t)tb;
+    pa.nrest = (uint8_t)rb;
+    pa.micro_bits = plan.passes[b].micro_bits;
+    pa.micro_begin = (uint32_t)plan.micro.size();
+    pa.micro_count = 0;
+    pa.scatter = 1;
+    auto tile_coords = [&](uint64_t x) {
+        uint64_t c = 0;
+        for (int k = 0; k < t; k++)
+            if ((x >> pa.load_pos[k]) & 1ull) c |= 1ull << k;
+        return c;
+    };
+    // basis of the span of the tile-bit images, starting with the low offsets
+    std::vector<uint64_t> out, coef, red, redc;  // red: reduced rows (pivot = lowest set bit)
+    auto add = [&](uint64_t v, uint64_t c) {
+        uint64_t x = v, xc = c;
+        for (size_t i = 0; i < red.size(); i++)
+            if ((x >> __builtin_ctzll(red[i])) & 1ull) {
+                x ^= red[i];
+                xc ^= redc[i];
+            }
+        if (!x) return false;
+        red.push_back(x);
+        redc.push_back(xc);
+        out.push_back(v);
+        coef.push_back(c);
+        return true;
+    };
+    for (int j = 0; j < low; j++)
+        if (!add(1ull << j, tile_coords(plan.trail_inv[(size_t)j]))) return;
+    for (int k = 0; k < t && (int)out.size() < t; k++) add(A[pa.load_pos[k]], 1ull << k);
+    if ((int)out.size() != t) return;
+    const uint32_t w = amp_bytes == 16 ? 4u : 5u;
+    ScatterArgs sa{};
+    for (int k = 0; k < t; k++) {
+        const uint32_t col = (uint32_t)coef[(size_t)k];
+        pa.store_cols[k] = (uint16_t)col;
+        pa.store_byte[k] = (col ^ ((col >> w) & ((1u << w) - 1u))) * (uint32_t)amp_bytes;
+        pa.store_pos[k] = pa.load_pos[k];
+        sa.out_tile[k] = out[(size_t)k];
+    }
+    for (int i = 0; i < rb; i++) sa.out_rest[i] = A[pa.rest_pos[i]];
+    // a run from |0...0> reads the input through the zero qubits of the first
+    // merged pass; the scatter pass launches every tile (it writes the other
+    // buffer, whose old contents must all be replaced)
+    const bool has_zero = plan.pass_zero.size() > b;
+    const uint64_t z = has_zero ? plan.pass_zero[b] : 0ull;
+    plan.passes.resize(b);
+    plan.pass_is_remap.resize(b);
+    if (has_zero) plan.pass_zero.resize(b);
+    plan.passes.push_back(pa);
+    plan.pass_is_remap.push_back(0);
+    if (has_zero) plan.pass_zero.push_back(z);
+    plan.gate_passes -= (int)(np - b) - 1;
+    plan.support_passes = std::min(plan.support_passes, (int)b);
+    plan.scatter_pass = (long)b;
+    plan.scatter_args = sa;
+}
+
 // Relative HBM time of a pass whose tiles are read and written in runs of
 // `run_bytes` contiguous bytes (one MI355X, 30-qubit passes: 64 B runs cost
 // 1.37x, 256 B runs 0.92x the time of 128 B runs).
@@ -1451,6 +1538,7 @@ std::string build_plan(const Program& prog, const PlanOptions& opt, Plan& plan)
         if (!e.empty()) return e;
     }
     plan_trailing(plan);
+    if (opt.scatter) make_scatter(plan, opt.amp_bytes);
     return "";
 }
 
@@ -1477,6 +1565,7 @@ std::string Plan::describe(bool verbose) const
             os << " direct=" << (p.fuse_first ? "first" : "") << (p.fuse_first && p.fuse_last ? "," : "")
                << (p.fuse_last ? "last" : "");
         if (p.light && p.micro_count) os << " light";
+        if (p.scatter) os << " scatter";
         // state bits known zero at this pass's input in a run from |0...0>
         // (its loads skip those amplitudes)
         if (i < pass_zero.size() && pass_zero[i]) os << " zero_in=" << __builtin_popcountll(pass_zero[i]);

@@ -100,12 +100,25 @@ struct PassArgs {
     // without a right diagonal, dense 1-qubit ops, phases, Hadamards,
     // CNOTs): the pass runs on the kernel variant that dispatches only those.
     uint8_t light;
+    // An out-of-place permutation pass (Plan::scatter): the tile is read from
+    // one state buffer and its amplitudes written to arbitrary 2^low-amplitude
+    // runs of the other.
+    uint8_t scatter;
     uint64_t first_g[16];
     uint64_t last_g[16];
     uint64_t synth_index;
     // Set per launch in a run from |0...0>: amplitudes whose state offset
     // meets zero_mask are known to be zero and are not read.
     uint64_t zero_mask;
+};
+
+// Store side of a scatter pass: store index bit k of a tile goes to state
+// offset out_tile[k], bit i of the tile number to out_rest[i] (offsets
+// combine by XOR); store index bits 0..low-1 are offset bits 0..low-1, so
+// every write is a whole run.
+struct ScatterArgs {
+    uint64_t out_tile[16];
+    uint64_t out_rest[kMaxQubits];
 };
 
 struct PlanGate {
@@ -149,6 +162,7 @@ struct PlanOptions {
     bool defer = true;       // execute() measures through the trailing permutation passes (simulator)
     bool split = true;       // dense 2-qubit gates that are CNOT x (A (x) B) products run as their factors
     bool reorder = true;     // swap-free programs: also plan the gates in a canonical order, keep the cheaper
+    bool scatter = false;    // 2+ trailing permutation passes become one out-of-place pass (needs a second buffer)
 };
 
 struct Plan {
@@ -177,6 +191,10 @@ struct Plan {
     // can read the state through it instead of running them.
     size_t trail_begin = 0;
     std::vector<uint64_t> trail_inv;
+    // The out-of-place permutation pass, when the trailing passes were
+    // merged into one (its index, -1 if none, and its store offsets).
+    long scatter_pass = -1;
+    ScatterArgs scatter_args{};
     std::string describe(bool verbose = false) const;
 };
 

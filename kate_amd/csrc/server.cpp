@@ -380,6 +380,7 @@ int kate_server_create(const kate_server_options* options, kate_server** out)
     kate::PlanOptions po;
     std::string err = kate::plan_options(ko, po);
     if (!err.empty()) return sfail(KATE_ERR_ARG, err);
+    po.scatter = false;  // one state buffer per request
     if (po.auto_tile) {
         // one tile size, planned once per request on the scheduler thread:
         // 12-qubit tiles (a mix of 128 circuits of 12..30 qubits at

@@ -158,6 +158,7 @@ struct kate_sim {
     kate::Plan plan;
     int nq = 0, nq_int = 0;
     void* d_state = nullptr;
+    void* d_state2 = nullptr;  // the other buffer of the scatter pass (which then swaps the two)
     kate::MicroStep* d_micro = nullptr;
     kate::MicroOp* d_ops = nullptr;
     void* d_coef = nullptr;
@@ -193,6 +194,7 @@ void release(kate_sim* s)
     (void)hipSetDevice(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     (void)hipFree(s->d_state);
+    (void)hipFree(s->d_state2);
     (void)hipFree(s->d_micro);
     (void)hipFree(s->d_ops);
     (void)hipFree(s->d_coef);
@@ -214,6 +216,18 @@ void release(kate_sim* s)
 // Passes [begin, end) -- all of them by default.
 // With `after` (events, one per pass in [begin, end)) an event is recorded
 // after each pass and `tiles` receives each pass's launched tile count.
+// One pass on the state; the scatter pass writes the other buffer, which
+// then holds the state.
+int launch_plan_pass(kate_sim* s, const kate::PassArgs& pa, const kate::DevPlan& dp)
+{
+    if (!pa.scatter) return kate::launch_pass(s->dtype, s->plan.tile_bits, s->d_state, s->nq_int, pa, dp, s->stream);
+    if (!s->d_state2) return (int)hipErrorInvalidValue;
+    int e = kate::launch_scatter(s->dtype, s->plan.tile_bits, s->d_state, s->d_state2, pa, s->plan.scatter_args,
+                                 s->stream);
+    if (!e) std::swap(s->d_state, s->d_state2);
+    return e;
+}
+
 int run_passes(kate_sim* s, bool from_zero = false, size_t begin = 0, size_t end = (size_t)-1,
                hipEvent_t* after = nullptr, uint64_t* tiles = nullptr)
 {
@@ -221,7 +235,7 @@ int run_passes(kate_sim* s, bool from_zero = false, size_t begin = 0, size_t end
     end = std::min(end, s->plan.passes.size());
     for (size_t i = begin; i < end; i++) {
         const kate::PassArgs pa = from_zero ? kate::zero_start_pass(s->plan, i, s->support, end) : s->plan.passes[i];
-        int e = kate::launch_pass(s->dtype, s->plan.tile_bits, s->d_state, s->nq_int, pa, dp, s->stream);
+        int e = launch_plan_pass(s, pa, dp);
         if (e) return fail(KATE_ERR_HIP, std::string("pass launch: ") + hipGetErrorString((hipError_t)e));
         if (tiles) tiles[i - begin] = 1ull << pa.nrest;
         if (after) KATE_HIP(hipEventRecord(after[i - begin], s->stream));
@@ -353,13 +367,14 @@ std::string plan_options(const kate_options& opt, kate::PlanOptions& po)
     po.amp_bytes = opt.dtype == KATE_C128 ? 16 : 8;
     if (opt.plan_flags &
         ~(KATE_PLAN_NO_EULER | KATE_PLAN_NO_DIRECT | KATE_PLAN_NO_SUPPORT | KATE_PLAN_NO_DEFER | KATE_PLAN_NO_SPLIT |
-          KATE_PLAN_NO_REORDER | KATE_PLAN_NO_FACTOR))
+          KATE_PLAN_NO_REORDER | KATE_PLAN_NO_FACTOR | KATE_PLAN_NO_SCATTER))
         return "unknown plan_flags bits";
     po.split = !(opt.plan_flags & KATE_PLAN_NO_SPLIT);
     po.reorder = !(opt.plan_flags & KATE_PLAN_NO_REORDER);
     po.defer = !(opt.plan_flags & KATE_PLAN_NO_DEFER);
     po.euler = !(opt.plan_flags & KATE_PLAN_NO_EULER);
     po.factor = !(opt.plan_flags & KATE_PLAN_NO_FACTOR);
+    po.scatter = !(opt.plan_flags & KATE_PLAN_NO_SCATTER);
     po.direct = !(opt.plan_flags & KATE_PLAN_NO_DIRECT);
     po.support = !(opt.plan_flags & KATE_PLAN_NO_SUPPORT);
     if (po.micro_bits != 3 && po.micro_bits != 4) return "micro_bits must be 3 or 4";
@@ -483,6 +498,16 @@ int kate_sim_create(const kate_program* program, const kate_options* options, ka
     }
     s->support = po.support;
     s->defer = po.defer;
+    if (po.scatter) {
+        // the scatter pass needs a second state buffer: only when the GPU
+        // has room for both and 1 GiB to spare
+        size_t fr = 0, tot = 0;
+        const int nmax = std::max(s->prog.nqubits, std::max(po.min_qubits, 13));
+        const unsigned __int128 need = ((unsigned __int128)2 << nmax) * s->amp_bytes + (1ull << 30);
+        if (nmax >= 60 || hipSetDevice(s->device) != hipSuccess || hipMemGetInfo(&fr, &tot) != hipSuccess ||
+            (unsigned __int128)fr < need)
+            po.scatter = false;
+    }
     err = kate::build_plan(s->prog, po, s->plan);
     if (!err.empty()) {
         delete s;
@@ -504,6 +529,7 @@ int kate_sim_create(const kate_program* program, const kate_options* options, ka
     step(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
     for (auto& e : s->ev) step(hipEventCreate(&e), "hipEventCreate");
     step(hipMalloc(&s->d_state, (1ull << s->nq_int) * s->amp_bytes), "hipMalloc(state)");
+    if (s->plan.scatter_pass >= 0) step(hipMalloc(&s->d_state2, (1ull << s->nq_int) * s->amp_bytes), "hipMalloc(state 2)");
     step(hipMalloc((void**)&s->d_scratch, kate::reduce_scratch_doubles(s->nq_int) * sizeof(double)), "hipMalloc(scratch)");
     step(hipMalloc((void**)&s->d_out, (2 + 2 * (size_t)s->nq_int + 64) * sizeof(double)), "hipMalloc(out)");
     step(hipMalloc((void**)&s->d_masks, 16 * sizeof(uint64_t)), "hipMalloc(masks)");
@@ -583,7 +609,7 @@ int kate_sim_run_timed(kate_sim* s, float* pass_ms, int32_t* pass_is_remap, int3
     kate::DevPlan dp{s->d_micro, s->d_ops, s->d_coef, s->d_raw};
     if (rc == KATE_OK && hipEventRecord(ev[0], s->stream) != hipSuccess) rc = fail(KATE_ERR_HIP, "hipEventRecord");
     for (size_t i = 0; i < np && rc == KATE_OK; i++) {
-        int e = kate::launch_pass(s->dtype, s->plan.tile_bits, s->d_state, s->nq_int, s->plan.passes[i], dp, s->stream);
+        int e = launch_plan_pass(s, s->plan.passes[i], dp);
         if (e) rc = fail(KATE_ERR_HIP, std::string("pass launch: ") + hipGetErrorString((hipError_t)e));
         else if (hipEventRecord(ev[i + 1], s->stream) != hipSuccess) rc = fail(KATE_ERR_HIP, "hipEventRecord");
     }
@@ -614,8 +640,13 @@ int kate_sim_copy_ceiling(kate_sim* s, float* pass_ms, int32_t cap, int32_t* npa
         if (rc == KATE_OK && hipEventCreate(&e) != hipSuccess) rc = fail(KATE_ERR_HIP, "hipEventCreate");
     if (rc == KATE_OK && hipEventRecord(ev[0], s->stream) != hipSuccess) rc = fail(KATE_ERR_HIP, "hipEventRecord");
     for (size_t i = 0; i < np && rc == KATE_OK; i++) {
-        int e = kate::launch_copy_tiles(s->dtype, s->plan.tile_bits, s->d_state, s->nq_int, s->plan.passes[i],
-                                        s->stream);
+        // the scatter pass is itself a plain copy: run it into the other
+        // buffer without making that the state
+        int e = s->plan.passes[i].scatter
+                    ? kate::launch_scatter(s->dtype, s->plan.tile_bits, s->d_state, s->d_state2, s->plan.passes[i],
+                                           s->plan.scatter_args, s->stream)
+                    : kate::launch_copy_tiles(s->dtype, s->plan.tile_bits, s->d_state, s->nq_int, s->plan.passes[i],
+                                              s->stream);
         if (e) rc = fail(KATE_ERR_HIP, std::string("copy launch: ") + hipGetErrorString((hipError_t)e));
         else if (hipEventRecord(ev[i + 1], s->stream) != hipSuccess) rc = fail(KATE_ERR_HIP, "hipEventRecord");
     }

@@ -233,6 +233,11 @@ private:
     {
         std::vector<double> tp(2 * (size_t)nqbits);
         check(kate_sim_execute(sim_, in, tp.data()));
+        // a plan that ends with an out-of-place pass leaves the state in the
+        // simulator's other buffer: gpu_qbits_states[0] follows it
+        void* d = nullptr;
+        check(kate_sim_device_state(sim_, &d));
+        state_ptr_[0] = static_cast<Complex*>(d);
         std::vector<std::pair<double, double>> v((size_t)nqbits);
         for (int i = 0; i < nqbits; i++) v[(size_t)i] = std::make_pair(tp[2 * (size_t)i], tp[2 * (size_t)i + 1]);
         if (displaytime) {

@@ -201,3 +201,69 @@ def test_copy_ceiling_leaves_the_state_unchanged(gpu, dtype):
         passes = sim.timings()["passes"]
     assert len(ms) == passes and np.all(ms > 0)
     assert np.array_equal(before, after)
+
+
+SCATTER_GEN = {
+    "layered20": (20, lambda n: W.layered(n, 3, 41)),
+    "layered18": (18, lambda n: W.layered(n, 4, 5)),
+    "mixed_tail": (16, lambda n: W.random_mixed(n, 100, 3) + [W.Gate(W.CNOT, (i + 1, i)) for i in range(n - 1)]),
+    "haar_tail": (22, lambda n: W.haar_layers(n, 2, 3) + [W.Gate(W.CNOT, (i + 1, i)) for i in range(n - 1)]),
+}
+# (dtype, tile, low, circuit): shapes whose plans end with 2+ permutation passes
+SCATTER_CASES = [("c128", 12, 3, "layered18"), ("c128", 12, 3, "mixed_tail"), ("c128", 10, 3, "layered20"),
+                 ("c128", 10, 3, "mixed_tail"), ("c128", 11, 3, "haar_tail"), ("c64", 13, 4, "layered20"),
+                 ("c64", 11, 3, "layered18"), ("c64", 11, 3, "mixed_tail")]
+
+
+@pytest.mark.parametrize("dtype,tile,low,name", SCATTER_CASES, ids=["-".join(map(str, c)) for c in SCATTER_CASES])
+def test_scatter_pass_bit_exact(gpu, dtype, tile, low, name):
+    """The trailing permutation passes merged into one out-of-place pass
+    (the state moves to the second buffer) give the same state bit for bit
+    as the passes run in place, with the gate passes pinned (tile, low
+    qubits, program order); and the oracle's to the usual tolerance."""
+    n, gen = SCATTER_GEN[name]
+    say("scatter", dtype, tile, low, name, n)
+    gates = gen(n)
+    opt = dict(tile_qubits=tile, low_qubits=low, reorder=False)
+    _, _, text = K.Circuit(n, gates).plan(dtype, **opt)
+    assert " scatter" in text, text.splitlines()[-1]
+    a = state_of(K.Circuit(n, gates), dtype, **opt)
+    b = state_of(K.Circuit(n, gates), dtype, scatter=False, **opt)
+    assert np.array_equal(a, b)
+    tol = 1e-12 if dtype == "c128" else 2e-6
+    assert np.max(np.abs(a - O.simulate(n, gates))) <= tol
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_scatter_pass_every_read_path(gpu, dtype):
+    """Around the buffer swap of the scatter pass: repeated execute(), the
+    deferred form (the scatter pass run by the next read), proba_state,
+    Z strings, the pass-by-pass run from a basis state and the copy-ceiling
+    timing (which must leave the state as it was) all agree with the plan
+    that runs its trailing passes in place."""
+    say("scatter read paths", dtype)
+    n = 19
+    gates = W.layered(n, 3, 8)
+    opt = dict(tile_qubits=12, low_qubits=3, reorder=False)
+    masks = [1, 3, (1 << n) - 1, 1 << 7]
+    with K.Simulator(K.Circuit(n, gates), dtype=dtype, **opt) as a, \
+            K.Simulator(K.Circuit(n, gates), dtype=dtype, scatter=False, **opt) as b, \
+            K.Simulator(K.Circuit(n, gates), dtype=dtype, defer=True, **opt) as d:
+        pb = b.execute()
+        sb = b.state()
+        for _ in range(3):
+            pa = a.execute()
+        assert np.array_equal(pa, pb)
+        assert np.array_equal(a.state(), sb)
+        assert np.array_equal(a.expect_z(masks), b.expect_z(masks))
+        assert np.array_equal(a.measure_proba(), b.measure_proba())
+        pd = d.execute()
+        assert np.max(np.abs(pd - pb)) <= (1e-12 if dtype == "c128" else 1e-5)
+        assert np.array_equal(d.state(), sb)
+        a.copy_ceiling()
+        assert np.array_equal(a.state(), sb)
+        a.init_basis(5)
+        b.init_basis(5)
+        a.run_timed()
+        b.run_timed()
+        assert np.array_equal(a.state(), b.state())
