@@ -364,58 +364,33 @@ __global__ __launch_bounds__(1 << (TB - 4)) void k_copy_tile(Cx<R>* __restrict__
 // are state offset bits 0..low-1, so each wave writes whole runs.
 template <typename R, int TB>
 __global__ __launch_bounds__(1 << (TB - 4)) void k_scatter(const Cx<R>* __restrict__ src, Cx<R>* __restrict__ dst,
-                                                          PassArgs pa, ScatterArgs sa)
+                                                          PassArgs pa, const ScatterArgs* __restrict__ sa)
 {
     constexpr int NB = 4, N = 1 << NB, NT = 1 << (TB - NB);
     __shared__ Cx<R> tile[1 << TB];
     const uint32_t tid = threadIdx.x;
     const uint64_t blk = blockIdx.x;
-    {
-        uint64_t tb = rest_base(pa, blk);
+    uint64_t tb = rest_base(pa, blk);
 #pragma unroll
-        for (int b = 0; b < TB - NB; b++) tb |= (uint64_t)((tid >> b) & 1u) << pa.load_pos[b];
-        Cx<R> r[N];
-#pragma unroll
-        for (int i = 0; i < N; i++) {
-            uint64_t off = tb;
-#pragma unroll
-            for (int k = 0; k < NB; k++)
-                if ((i >> k) & 1) off |= 1ull << pa.load_pos[TB - NB + k];
-            r[i] = pa.synth_basis ? synth_amp<R>(pa, off)
-                                  : ((off & pa.zero_mask) ? Cx<R>{(R)0, (R)0} : src[off]);
-        }
-        const uint32_t tsw = swz<R>(tid) * (uint32_t)sizeof(Cx<R>);
-#pragma unroll
-        for (int i = 0; i < N; i++) lds_at(tile, tsw ^ (swz<R>((uint32_t)i * NT) * (uint32_t)sizeof(Cx<R>))) = r[i];
-    }
-    __syncthreads();
-    uint64_t ob = 0;
-#pragma unroll
-    for (int i = 0; i < kMaxQubits - 10; i++)
-        if (i < pa.nrest && ((blk >> i) & 1ull)) ob ^= sa.out_rest[i];
-    uint32_t lt = 0;
-#pragma unroll
-    for (int b = 0; b < TB - NB; b++)
-        if ((tid >> b) & 1u) {
-            lt ^= pa.store_byte[b];
-            ob ^= sa.out_tile[b];
-        }
+    for (int b = 0; b < TB - NB; b++) tb |= (uint64_t)((tid >> b) & 1u) << pa.load_pos[b];
+    Cx<R> r[N];
 #pragma unroll
     for (int i = 0; i < N; i++) {
-        uint32_t l = lt;
-        uint64_t o = ob;
+        uint64_t off = tb;
 #pragma unroll
         for (int k = 0; k < NB; k++)
-            if ((i >> k) & 1) {
-                l ^= pa.store_byte[TB - NB + k];
-                o ^= sa.out_tile[TB - NB + k];
-            }
-        dst[o] = lds_at(tile, l);
+            if ((i >> k) & 1) off |= 1ull << pa.load_pos[TB - NB + k];
+        r[i] = pa.synth_basis ? synth_amp<R>(pa, off) : ((off & pa.zero_mask) ? Cx<R>{(R)0, (R)0} : src[off]);
     }
+    const uint32_t tsw = swz<R>(tid) * (uint32_t)sizeof(Cx<R>);
+#pragma unroll
+    for (int i = 0; i < N; i++) lds_at(tile, tsw ^ (swz<R>((uint32_t)i * NT) * (uint32_t)sizeof(Cx<R>))) = r[i];
+    __syncthreads();
+    scatter_store<R, TB, NB>(dst, pa, blk, tile, sa);
 }
 
 template <typename R>
-int scatter_r(int tile_bits, const void* src, void* dst, const PassArgs& pa, const ScatterArgs& sa, hipStream_t s)
+int scatter_r(int tile_bits, const void* src, void* dst, const PassArgs& pa, const ScatterArgs* sa, hipStream_t s)
 {
     const unsigned grid = 1u << pa.nrest;
     switch (tile_bits) {
@@ -516,10 +491,10 @@ int launch_pass(int dtype, int tile_bits, void* state, int nq_int, const PassArg
     return launch_pass_f32(tile_bits, state, nq_int, pa, dp, stream);
 }
 
-int launch_scatter(int dtype, int tile_bits, const void* src, void* dst, const PassArgs& pa, const ScatterArgs& sa,
+int launch_scatter(int dtype, int tile_bits, const void* src, void* dst, const PassArgs& pa, const ScatterArgs* sa,
                    void* stream)
 {
-    if (!pa.scatter || pa.micro_count || pa.tile_bits != tile_bits) return (int)hipErrorInvalidValue;
+    if (!pa.scatter || pa.tile_bits != tile_bits || !sa || !dst) return (int)hipErrorInvalidValue;
     if (dtype == 0) return scatter_r<double>(tile_bits, src, dst, pa, sa, (hipStream_t)stream);
     return scatter_r<float>(tile_bits, src, dst, pa, sa, (hipStream_t)stream);
 }

@@ -14,6 +14,8 @@ struct DevPlan {
     const MicroOp* ops;
     const void* coef;  // complex of the state's precision
     const uint32_t* raw;  // matrix-core operands
+    void* dst = nullptr;                  // a gate pass with PassArgs::scatter writes this buffer
+    const ScatterArgs* scatter = nullptr; // through these offsets (device memory)
 };
 
 struct BatchTask {
@@ -62,8 +64,9 @@ int launch_pass(int dtype, int tile_bits, void* state, int nq_int, const PassArg
 // (load positions, no arithmetic, no LDS): the HBM time of its access pattern.
 int launch_copy_tiles(int dtype, int tile_bits, void* state, int nq_int, const PassArgs& pa, void* stream);
 // The out-of-place permutation pass (Plan::scatter_pass): every tile of `src`
-// to runs of `dst`.
-int launch_scatter(int dtype, int tile_bits, const void* src, void* dst, const PassArgs& pa, const ScatterArgs& sa,
+// to runs of `dst` through `sa` (device memory); for a gate pass with
+// PassArgs::scatter, a plain copy through its store (the copy ceiling).
+int launch_scatter(int dtype, int tile_bits, const void* src, void* dst, const PassArgs& pa, const ScatterArgs* sa,
                    void* stream);
 // |index> (basis state).
 int launch_init_basis(int dtype, void* state, int nq_int, uint64_t index, void* stream);

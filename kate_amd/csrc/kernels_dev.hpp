@@ -778,10 +778,48 @@ __device__ __forceinline__ void store_tile(Cx<R>* __restrict__ state, const Pass
 // (its bits are the non-tile qubits rest_pos[]).  Each of the 2^(TB-NB)
 // threads holds N = 2^NB amplitudes; the register bits are tile bits
 // TB-NB..TB-1 when loading and storing.
+// Store of a gate pass that also applies the trailing permutation of its
+// circuit (PassArgs::scatter): store index u (thread bits, then register
+// bits) reads LDS at the XOR of store_byte[k] over its set bits and writes
+// `dst` at the XOR of out_tile[k] and of out_rest[i] over the set bits of
+// the tile number; store index bits 0..low-1 are offset bits 0..low-1, so
+// the writes are whole runs.
+template <typename R, int TB, int NB>
+__device__ __forceinline__ void scatter_store(Cx<R>* __restrict__ dst, const PassArgs& pa, uint64_t blk, Cx<R>* tile,
+                                              const ScatterArgs* __restrict__ sa)
+{
+    constexpr int N = 1 << NB;
+    const uint32_t tid = threadIdx.x;
+    uint64_t ob = 0;
+#pragma unroll
+    for (int i = 0; i < kMaxQubits - 10; i++)
+        if (i < pa.nrest && ((blk >> i) & 1ull)) ob ^= sa->out_rest[i];
+    uint32_t lt = 0;
+#pragma unroll
+    for (int b = 0; b < TB - NB; b++)
+        if ((tid >> b) & 1u) {
+            lt ^= pa.store_byte[b];
+            ob ^= sa->out_tile[b];
+        }
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        uint32_t l = lt;
+        uint64_t o = ob;
+#pragma unroll
+        for (int k = 0; k < NB; k++)
+            if ((i >> k) & 1) {
+                l ^= pa.store_byte[TB - NB + k];
+                o ^= sa->out_tile[TB - NB + k];
+            }
+        dst[o] = lds_at(tile, l);
+    }
+}
+
 template <typename R, int TB, bool WIDE, int NB, bool LIGHT>
 __device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassArgs& pa, uint64_t blk, Cx<R>* tile,
                                           const MicroStep* __restrict__ micro, const MicroOp* __restrict__ ops,
-                                          const Cx<R>* __restrict__ coef, const uint32_t* __restrict__ raw)
+                                          const Cx<R>* __restrict__ coef, const uint32_t* __restrict__ raw,
+                                          Cx<R>* __restrict__ dst, const ScatterArgs* __restrict__ sa)
 {
     constexpr int NT = 1 << (TB - NB);
     constexpr int N = 1 << NB;
@@ -881,6 +919,8 @@ __device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassA
                             direct_reg_offset<NB>(pa.last_g, (int)f);
 #pragma unroll
         for (int s = 0; s < N; s++) state[gs ^ direct_reg_offset<NB>(pa.last_g, s)] = v[s];
+    } else if (pa.scatter) {
+        scatter_store<R, TB, NB>(dst, pa, blk, tile, sa);
     } else {
         store_tile<R, TB, NB>(state, pa, base, tile);
     }
@@ -909,10 +949,12 @@ void k_pass(Cx<R>* __restrict__ state, PassArgs pa,
                                                          const MicroStep* __restrict__ micro,
                                                          const MicroOp* __restrict__ ops,
                                                          const Cx<R>* __restrict__ coef,
-                                                         const uint32_t* __restrict__ raw)
+                                                         const uint32_t* __restrict__ raw,
+                                                         Cx<R>* __restrict__ dst,
+                                                         const ScatterArgs* __restrict__ sa)
 {
     __shared__ Cx<R> tile[1 << TB];
-    tile_pass<R, TB, WIDE, NB, LIGHT>(state, pa, blockIdx.x, tile, micro, ops, coef, raw);
+    tile_pass<R, TB, WIDE, NB, LIGHT>(state, pa, blockIdx.x, tile, micro, ops, coef, raw, dst, sa);
 }
 
 // ---- many circuits in one persistent launch ---------------------------------
@@ -989,7 +1031,7 @@ void k_batch(Cx<R>* __restrict__ states,
             __syncthreads();
         }
         tile_pass<R, TB, WIDE, NB, false>(states + circ_off[task.circuit], passes[task.pass], task.tile, tile, micro, ops,
-                                   coef, raw);
+                                          coef, raw, nullptr, nullptr);
         // every wave's stores of this tile have completed (each wave drains
         // its memory counters; a workgroup-scope barrier alone does not wait
         // for global stores) before thread 0 fences and counts the task, and
@@ -1010,13 +1052,13 @@ int pass_t(void* state, int nq_int, const PassArgs& pa, const DevPlan& dp, hipSt
     const unsigned grid = 1u << pa.nrest;
     if (pa.has_wide)
         hipLaunchKernelGGL((k_pass<R, TB, true, NB, false>), dim3(grid), dim3(1 << (TB - NB)), 0, s, (Cx<R>*)state,
-                           pa, dp.micro, dp.ops, (const Cx<R>*)dp.coef, dp.raw);
+                           pa, dp.micro, dp.ops, (const Cx<R>*)dp.coef, dp.raw, (Cx<R>*)dp.dst, dp.scatter);
     else if (pa.light)
         hipLaunchKernelGGL((k_pass<R, TB, false, NB, true>), dim3(grid), dim3(1 << (TB - NB)), 0, s, (Cx<R>*)state,
-                           pa, dp.micro, dp.ops, (const Cx<R>*)dp.coef, dp.raw);
+                           pa, dp.micro, dp.ops, (const Cx<R>*)dp.coef, dp.raw, (Cx<R>*)dp.dst, dp.scatter);
     else
         hipLaunchKernelGGL((k_pass<R, TB, false, NB, false>), dim3(grid), dim3(1 << (TB - NB)), 0, s, (Cx<R>*)state,
-                           pa, dp.micro, dp.ops, (const Cx<R>*)dp.coef, dp.raw);
+                           pa, dp.micro, dp.ops, (const Cx<R>*)dp.coef, dp.raw, (Cx<R>*)dp.dst, dp.scatter);
     return (int)hipGetLastError();
 }
 

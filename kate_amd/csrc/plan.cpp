@@ -1281,8 +1281,10 @@ static void plan_trailing(Plan& plan)
     }
 }
 
-// Two or more trailing permutation passes (together the map A, known as
-// trail_inv = A^-1) as one out-of-place pass.  Its tile holds the low
+// The trailing permutation passes (together the map A, known as
+// trail_inv = A^-1) folded into the last gate pass's store when its tile
+// holds every qubit that A^-1 of a low-qubit offset involves; else, when
+// there are two or more, as one out-of-place pass.  That pass's tile holds the low
 // qubits and every qubit that A^-1 of a low-qubit offset involves, so that
 // each output run of 2^low amplitudes has all its sources in one tile; the
 // tile is read in runs as usual and written through A: the store index has
@@ -1292,7 +1294,7 @@ static void plan_trailing(Plan& plan)
 static void make_scatter(Plan& plan, int amp_bytes)
 {
     const size_t b = plan.trail_begin, np = plan.passes.size();
-    if (np < b + 2 || plan.trail_inv.empty() || plan.remap_passes) return;
+    if (np <= b || plan.trail_inv.empty() || plan.remap_passes) return;
     for (size_t k = b; k < np; k++)
         if (plan.pass_is_remap[k]) return;
     const int n = plan.nq_int, t = plan.tile_bits, low = std::min(plan.low_qubits, plan.tile_bits);
@@ -1301,7 +1303,66 @@ static void make_scatter(Plan& plan, int amp_bytes)
     if (!gf2_inverse(plan.trail_inv, A)) return;
     uint64_t set = (1ull << low) - 1ull;
     for (int j = 0; j < low; j++) set |= plan.trail_inv[(size_t)j];
-    if (popc(set) > t) return;
+    const uint32_t w = amp_bytes == 16 ? 4u : 5u;
+    auto lds_bytes = [&](uint32_t col) { return (col ^ ((col >> w) & ((1u << w) - 1u))) * (uint32_t)amp_bytes; };
+    // When the last gate pass's tile holds those qubits, that pass itself
+    // stores through A (out of place), and no permutation pass remains.
+    if (b > 0 && !plan.pass_is_remap[b - 1] && plan.passes[b - 1].micro_count > 0) {
+        PassArgs& gp = plan.passes[b - 1];
+        uint64_t spos = 0;
+        for (int p = 0; p < gp.tile_bits; p++) spos |= 1ull << gp.store_pos[p];
+        if ((set & ~spos) == 0 && gp.tile_bits == t) {
+            auto store_coords = [&](uint64_t x) {
+                uint64_t c = 0;
+                for (int p = 0; p < t; p++)
+                    if ((x >> gp.store_pos[p]) & 1ull) c |= 1ull << p;
+                return c;
+            };
+            std::vector<uint64_t> out, coef, red;
+            auto add = [&](uint64_t v, uint64_t c) {
+                uint64_t x = v;
+                for (uint64_t r : red)
+                    if ((x >> __builtin_ctzll(r)) & 1ull) x ^= r;
+                if (!x) return false;
+                red.push_back(x);
+                out.push_back(v);
+                coef.push_back(c);
+                return true;
+            };
+            bool ok = true;
+            for (int j = 0; j < low && ok; j++) ok = add(1ull << j, store_coords(plan.trail_inv[(size_t)j]));
+            for (int p = 0; p < t && ok && (int)out.size() < t; p++) add(A[gp.store_pos[p]], 1ull << p);
+            if (ok && (int)out.size() == t) {
+                ScatterArgs sa{};
+                uint16_t cols[16];
+                for (int k = 0; k < t; k++) {
+                    uint32_t col = 0;
+                    for (int p = 0; p < t; p++)
+                        if ((coef[(size_t)k] >> p) & 1ull) col ^= gp.store_cols[p];
+                    cols[k] = (uint16_t)col;
+                    sa.out_tile[k] = out[(size_t)k];
+                }
+                for (int k = 0; k < t; k++) {
+                    gp.store_cols[k] = cols[k];
+                    gp.store_byte[k] = lds_bytes(cols[k]);
+                }
+                for (int i = 0; i < gp.nrest; i++) sa.out_rest[i] = A[gp.rest_pos[i]];
+                gp.scatter = 1;
+                gp.fuse_last = 0;  // the last micro-step writes LDS, the scatter store reads it
+                plan.passes.resize(b);
+                plan.pass_is_remap.resize(b);
+                if (plan.pass_zero.size() > b) plan.pass_zero.resize(b);
+                plan.gate_passes -= (int)(np - b);
+                plan.support_passes = std::min(plan.support_passes, (int)b - 1);
+                plan.scatter_pass = (long)(b - 1);
+                plan.scatter_args = sa;
+                plan.trail_begin = plan.passes.size();
+                plan.trail_inv.clear();
+                return;
+            }
+        }
+    }
+    if (np < b + 2 || popc(set) > t) return;  // one pass stays as it is
     set = pad_set(set, t, n);
     PassArgs pa{};
     int tb = 0, rb = 0;
@@ -1341,12 +1402,11 @@ static void make_scatter(Plan& plan, int amp_bytes)
         if (!add(1ull << j, tile_coords(plan.trail_inv[(size_t)j]))) return;
     for (int k = 0; k < t && (int)out.size() < t; k++) add(A[pa.load_pos[k]], 1ull << k);
     if ((int)out.size() != t) return;
-    const uint32_t w = amp_bytes == 16 ? 4u : 5u;
     ScatterArgs sa{};
     for (int k = 0; k < t; k++) {
         const uint32_t col = (uint32_t)coef[(size_t)k];
         pa.store_cols[k] = (uint16_t)col;
-        pa.store_byte[k] = (col ^ ((col >> w) & ((1u << w) - 1u))) * (uint32_t)amp_bytes;
+        pa.store_byte[k] = lds_bytes(col);
         pa.store_pos[k] = pa.load_pos[k];
         sa.out_tile[k] = out[(size_t)k];
     }

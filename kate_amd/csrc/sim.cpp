@@ -159,6 +159,7 @@ struct kate_sim {
     int nq = 0, nq_int = 0;
     void* d_state = nullptr;
     void* d_state2 = nullptr;  // the other buffer of the scatter pass (which then swaps the two)
+    kate::ScatterArgs* d_scatter = nullptr;  // that pass's store offsets
     kate::MicroStep* d_micro = nullptr;
     kate::MicroOp* d_ops = nullptr;
     void* d_coef = nullptr;
@@ -195,6 +196,7 @@ void release(kate_sim* s)
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     (void)hipFree(s->d_state);
     (void)hipFree(s->d_state2);
+    (void)hipFree(s->d_scatter);
     (void)hipFree(s->d_micro);
     (void)hipFree(s->d_ops);
     (void)hipFree(s->d_coef);
@@ -221,9 +223,16 @@ void release(kate_sim* s)
 int launch_plan_pass(kate_sim* s, const kate::PassArgs& pa, const kate::DevPlan& dp)
 {
     if (!pa.scatter) return kate::launch_pass(s->dtype, s->plan.tile_bits, s->d_state, s->nq_int, pa, dp, s->stream);
-    if (!s->d_state2) return (int)hipErrorInvalidValue;
-    int e = kate::launch_scatter(s->dtype, s->plan.tile_bits, s->d_state, s->d_state2, pa, s->plan.scatter_args,
-                                 s->stream);
+    if (!s->d_state2 || !s->d_scatter) return (int)hipErrorInvalidValue;
+    int e;
+    if (pa.micro_count) {  // a gate pass that stores through the trailing permutation
+        kate::DevPlan d = dp;
+        d.dst = s->d_state2;
+        d.scatter = s->d_scatter;
+        e = kate::launch_pass(s->dtype, s->plan.tile_bits, s->d_state, s->nq_int, pa, d, s->stream);
+    } else {
+        e = kate::launch_scatter(s->dtype, s->plan.tile_bits, s->d_state, s->d_state2, pa, s->d_scatter, s->stream);
+    }
     if (!e) std::swap(s->d_state, s->d_state2);
     return e;
 }
@@ -529,7 +538,13 @@ int kate_sim_create(const kate_program* program, const kate_options* options, ka
     step(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
     for (auto& e : s->ev) step(hipEventCreate(&e), "hipEventCreate");
     step(hipMalloc(&s->d_state, (1ull << s->nq_int) * s->amp_bytes), "hipMalloc(state)");
-    if (s->plan.scatter_pass >= 0) step(hipMalloc(&s->d_state2, (1ull << s->nq_int) * s->amp_bytes), "hipMalloc(state 2)");
+    if (s->plan.scatter_pass >= 0) {
+        step(hipMalloc(&s->d_state2, (1ull << s->nq_int) * s->amp_bytes), "hipMalloc(state 2)");
+        step(hipMalloc((void**)&s->d_scatter, sizeof(kate::ScatterArgs)), "hipMalloc(scatter)");
+        if (rc == KATE_OK)
+            step(hipMemcpy(s->d_scatter, &s->plan.scatter_args, sizeof(kate::ScatterArgs), hipMemcpyHostToDevice),
+                 "upload(scatter)");
+    }
     step(hipMalloc((void**)&s->d_scratch, kate::reduce_scratch_doubles(s->nq_int) * sizeof(double)), "hipMalloc(scratch)");
     step(hipMalloc((void**)&s->d_out, (2 + 2 * (size_t)s->nq_int + 64) * sizeof(double)), "hipMalloc(out)");
     step(hipMalloc((void**)&s->d_masks, 16 * sizeof(uint64_t)), "hipMalloc(masks)");
@@ -640,11 +655,11 @@ int kate_sim_copy_ceiling(kate_sim* s, float* pass_ms, int32_t cap, int32_t* npa
         if (rc == KATE_OK && hipEventCreate(&e) != hipSuccess) rc = fail(KATE_ERR_HIP, "hipEventCreate");
     if (rc == KATE_OK && hipEventRecord(ev[0], s->stream) != hipSuccess) rc = fail(KATE_ERR_HIP, "hipEventRecord");
     for (size_t i = 0; i < np && rc == KATE_OK; i++) {
-        // the scatter pass is itself a plain copy: run it into the other
-        // buffer without making that the state
+        // a scatter pass is timed as a plain copy through its own store, into
+        // the other buffer, which does not become the state
         int e = s->plan.passes[i].scatter
                     ? kate::launch_scatter(s->dtype, s->plan.tile_bits, s->d_state, s->d_state2, s->plan.passes[i],
-                                           s->plan.scatter_args, s->stream)
+                                           s->d_scatter, s->stream)
                     : kate::launch_copy_tiles(s->dtype, s->plan.tile_bits, s->d_state, s->nq_int, s->plan.passes[i],
                                               s->stream);
         if (e) rc = fail(KATE_ERR_HIP, std::string("copy launch: ") + hipGetErrorString((hipError_t)e));
