@@ -31,12 +31,12 @@ HBM_PEAK_GBS = 8000.0
 # execute() that read and write every amplitude), from rocprofv3 FETCH_SIZE
 # and WRITE_SIZE runs of `bench.py --passes-only execute`, each scaled by what
 # the same counter reads on the k_copy_tile launches of that run (16 GiB read
-# + 16 GiB written per launch: factors 2.000 and 1.000): 1.00001 x the
+# + 16 GiB written per launch: factors 2.000 and 1.000): 1.0003 x the
 # algorithmic bytes (profiles/k_pass_f64_30q_layered_hbm_counters.md).  The
 # ratio holds for the plan it was measured on; the bench reports it only when
 # the current plan of the default workload is that plan.
-TRAFFIC_RATIO_30Q_LAYERED = 1.00001
-TRAFFIC_PLAN_30Q_LAYERED = "nq_int=30 tile_bits=12 low=3 passes=25 (gate 25, remap 0) micro=151 ops=199"
+TRAFFIC_RATIO_30Q_LAYERED = 1.0003
+TRAFFIC_PLAN_30Q_LAYERED = "nq_int=30 tile_bits=12 low=3 passes=24 (gate 24, remap 0) micro=151 ops=199"
 METRIC = "30q circuit ms + HBM % peak; served circuits/s + p99 at fidelity≥1-1e-6"
 
 
