@@ -140,7 +140,14 @@ enum {
        pass into a second state buffer (which is allocated only when the
        GPU has room for both states and 1 GiB more; results bit for bit the
        same) */
-    KATE_PLAN_NO_SCATTER = 128
+    KATE_PLAN_NO_SCATTER = 128,
+    /* keep every pass in place on the virtual qubit order instead of, from
+       the first pass after which no qubit is known zero, reading each tile
+       from the lowest state bits and writing (out of place, into the second
+       buffer) where the next pass reads (the last pass to virtual order;
+       needs the second buffer as KATE_PLAN_NO_SCATTER describes; results bit
+       for bit the same) */
+    KATE_PLAN_NO_RELABEL = 256
 };
 
 /* Wall-clock split of the last execute, as Kate's execute(true) prints it

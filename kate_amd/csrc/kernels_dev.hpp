@@ -784,16 +784,25 @@ __device__ __forceinline__ void store_tile(Cx<R>* __restrict__ state, const Pass
 // `dst` at the XOR of out_tile[k] and of out_rest[i] over the set bits of
 // the tile number; store index bits 0..low-1 are offset bits 0..low-1, so
 // the writes are whole runs.
+// Output offset of tile `blk` of an out-of-place pass: the XOR of
+// out_rest[i] over the set bits of blk (scalar loads, SALU).
+__device__ __forceinline__ uint64_t out_base(const PassArgs& pa, uint64_t blk, const ScatterArgs* __restrict__ sa)
+{
+    uint64_t ob = 0;
+#pragma unroll
+    for (int i = 0; i < kMaxQubits - 10; i++)
+        if (i < pa.nrest && ((blk >> i) & 1ull)) ob ^= sa->out_rest[i];
+    return ob;
+}
+
 template <typename R, int TB, int NB>
 __device__ __forceinline__ void scatter_store(Cx<R>* __restrict__ dst, const PassArgs& pa, uint64_t blk, Cx<R>* tile,
                                               const ScatterArgs* __restrict__ sa)
 {
     constexpr int N = 1 << NB;
     const uint32_t tid = threadIdx.x;
-    uint64_t ob = 0;
-#pragma unroll
-    for (int i = 0; i < kMaxQubits - 10; i++)
-        if (i < pa.nrest && ((blk >> i) & 1ull)) ob ^= sa->out_rest[i];
+    sa += pa.scatter_idx;
+    uint64_t ob = out_base(pa, blk, sa);
     uint32_t lt = 0;
 #pragma unroll
     for (int b = 0; b < TB - NB; b++)
@@ -853,10 +862,12 @@ __device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassA
             // between threads): every load of the tile has completed (its
             // value was used above) once all threads pass this barrier
             __syncthreads();
-            const uint64_t gs = base ^ direct_thread_offset<TB, NB>(pa.last_g, tid) ^
+            Cx<R>* __restrict__ out = pa.scatter ? dst : state;
+            const uint64_t ob = pa.scatter ? out_base(pa, blk, sa + pa.scatter_idx) : base;
+            const uint64_t gs = ob ^ direct_thread_offset<TB, NB>(pa.last_g, tid) ^
                                 direct_reg_offset<NB>(pa.last_g, (int)f);
 #pragma unroll
-            for (int s = 0; s < N; s++) state[gs ^ direct_reg_offset<NB>(pa.last_g, s)] = v[s];
+            for (int s = 0; s < N; s++) out[gs ^ direct_reg_offset<NB>(pa.last_g, s)] = v[s];
             return;
         }
         const uint32_t bw = b ^ reg_offset<NB>((int)f, oc);
@@ -915,10 +926,13 @@ __device__ __forceinline__ void tile_pass(Cx<R>* __restrict__ state, const PassA
 #pragma unroll
         for (int s = 0; s < N; s++) v[s] = lds_at(tile, b ^ reg_offset<NB>(s, oc));
         const uint32_t f = reg_step_ops<R, N, LIGHT>(v, msp, op_begin, op_count, ops, coef);
-        const uint64_t gs = base ^ direct_thread_offset<TB, NB>(pa.last_g, tid) ^
+        // an out-of-place pass writes the other buffer, at its own tile base
+        Cx<R>* __restrict__ out = pa.scatter ? dst : state;
+        const uint64_t ob = pa.scatter ? out_base(pa, blk, sa + pa.scatter_idx) : base;
+        const uint64_t gs = ob ^ direct_thread_offset<TB, NB>(pa.last_g, tid) ^
                             direct_reg_offset<NB>(pa.last_g, (int)f);
 #pragma unroll
-        for (int s = 0; s < N; s++) state[gs ^ direct_reg_offset<NB>(pa.last_g, s)] = v[s];
+        for (int s = 0; s < N; s++) out[gs ^ direct_reg_offset<NB>(pa.last_g, s)] = v[s];
     } else if (pa.scatter) {
         scatter_store<R, TB, NB>(dst, pa, blk, tile, sa);
     } else {

@@ -1348,6 +1348,7 @@ static void make_scatter(Plan& plan, int amp_bytes)
                 }
                 for (int i = 0; i < gp.nrest; i++) sa.out_rest[i] = A[gp.rest_pos[i]];
                 gp.scatter = 1;
+                gp.scatter_idx = (uint32_t)plan.scatter_args.size();
                 gp.fuse_last = 0;  // the last micro-step writes LDS, the scatter store reads it
                 plan.passes.resize(b);
                 plan.pass_is_remap.resize(b);
@@ -1355,7 +1356,7 @@ static void make_scatter(Plan& plan, int amp_bytes)
                 plan.gate_passes -= (int)(np - b);
                 plan.support_passes = std::min(plan.support_passes, (int)b - 1);
                 plan.scatter_pass = (long)(b - 1);
-                plan.scatter_args = sa;
+                plan.scatter_args.push_back(sa);
                 plan.trail_begin = plan.passes.size();
                 plan.trail_inv.clear();
                 return;
@@ -1376,6 +1377,7 @@ static void make_scatter(Plan& plan, int amp_bytes)
     pa.micro_begin = (uint32_t)plan.micro.size();
     pa.micro_count = 0;
     pa.scatter = 1;
+    pa.scatter_idx = (uint32_t)plan.scatter_args.size();
     auto tile_coords = [&](uint64_t x) {
         uint64_t c = 0;
         for (int k = 0; k < t; k++)
@@ -1425,7 +1427,84 @@ static void make_scatter(Plan& plan, int amp_bytes)
     plan.gate_passes -= (int)(np - b) - 1;
     plan.support_passes = std::min(plan.support_passes, (int)b);
     plan.scatter_pass = (long)b;
-    plan.scatter_args = sa;
+    plan.scatter_args.push_back(sa);
+}
+
+// Relabelled passes.  A pass whose tile qubits sit at state bits
+// 0..tile_bits-1 reads whole 64 KiB blocks; one whose tile qubits are high
+// bits reads 2^low-amplitude runs far apart, which the memory system serves
+// more slowly (one MI355X, complex128, 30 qubits: an out-of-place copy of
+// tiles over qubits {0,1,2,18..26} 8.3 ms, reading contiguous tiles and
+// writing to those positions 6.7 ms).  From the first pass after which no
+// qubit is still known zero (earlier passes skip tiles in place), every pass
+// is out of place: it reads its tile from bits 0..tile_bits-1 and writes its
+// amplitudes where the next pass's tile and rest qubits are placed (the last
+// one to virtual order, or through the trailing permutation).  Only physical
+// positions change; the arithmetic is that of the plain plan, bit for bit.
+static uint64_t permute_bits(const std::vector<int>& to, uint64_t x)
+{
+    uint64_t y = 0;
+    while (x) {
+        const int q = __builtin_ctzll(x);
+        x &= x - 1;
+        y |= 1ull << to[(size_t)q];
+    }
+    return y;
+}
+
+static void make_relabel(Plan& plan)
+{
+    const int n = plan.nq_int, t = plan.tile_bits;
+    const size_t np = plan.passes.size();
+    if (n > 64 || plan.remap_passes || np < 3 || plan.pass_zero.size() < np) return;
+    size_t start = np;
+    while (start > 0 && plan.pass_zero[start - 1] == 0) start--;
+    if (start == 0 || start + 2 > np) return;
+    for (size_t k = start; k < np; k++)
+        if (plan.pass_is_remap[k] || plan.passes[k].tile_bits != t || (k + 1 < np && plan.passes[k].scatter)) return;
+    // every pass must read and (unless it already scatters) write each state
+    // bit exactly once, so that the relabelling maps are permutations
+    auto covers = [&](const uint8_t* pos, const PassArgs& pa) {
+        if (pa.nrest != n - t) return false;
+        uint64_t seen = 0;
+        for (int b = 0; b < t; b++) seen |= 1ull << pos[b];
+        for (int i = 0; i < pa.nrest; i++) seen |= 1ull << pa.rest_pos[i];
+        return seen == (n == 64 ? ~0ull : (1ull << n) - 1) && __builtin_popcountll(seen) == n;
+    };
+    for (size_t k = start; k < np; k++) {
+        const PassArgs& pa = plan.passes[k];
+        if (!covers(pa.load_pos, pa) || (!pa.scatter && !covers(pa.store_pos, pa))) return;
+    }
+    std::vector<int> P((size_t)n), Q((size_t)n);
+    for (int q = 0; q < n; q++) P[(size_t)q] = q;
+    for (size_t k = start; k < np; k++) {
+        PassArgs& pa = plan.passes[k];
+        if (k + 1 < np) {
+            const PassArgs& nx = plan.passes[k + 1];
+            for (int b = 0; b < t; b++) Q[nx.load_pos[b]] = b;
+            for (int i = 0; i < nx.nrest; i++) Q[nx.rest_pos[i]] = t + i;
+        } else {
+            for (int q = 0; q < n; q++) Q[(size_t)q] = q;
+        }
+        if (!pa.scatter) {
+            ScatterArgs sa{};
+            for (int p = 0; p < t; p++) sa.out_tile[p] = 1ull << Q[pa.store_pos[p]];
+            for (int i = 0; i < pa.nrest; i++) sa.out_rest[i] = 1ull << Q[pa.rest_pos[i]];
+            for (int i = 0; i < 16; i++) pa.last_g[i] = permute_bits(Q, pa.last_g[i]);
+            pa.scatter = 1;
+            pa.scatter_idx = (uint32_t)plan.scatter_args.size();
+            plan.scatter_args.push_back(sa);
+        }
+        for (int b = 0; b < t; b++) pa.load_pos[b] = (uint8_t)P[pa.load_pos[b]];
+        for (int i = 0; i < pa.nrest; i++) pa.rest_pos[i] = (uint8_t)P[pa.rest_pos[i]];
+        for (int i = 0; i < 16; i++) pa.first_g[i] = permute_bits(P, pa.first_g[i]);
+        P = Q;
+    }
+    plan.support_passes = std::min(plan.support_passes, (int)start);
+    plan.relabel_begin = (long)start;
+    // execute() cannot measure through trailing passes of a relabelled plan
+    plan.trail_begin = np;
+    plan.trail_inv.clear();
 }
 
 // Relative HBM time of a pass whose tiles are read and written in runs of
@@ -1599,6 +1678,7 @@ std::string build_plan(const Program& prog, const PlanOptions& opt, Plan& plan)
     }
     plan_trailing(plan);
     if (opt.scatter) make_scatter(plan, opt.amp_bytes);
+    if (opt.scatter && opt.relabel) make_relabel(plan);
     return "";
 }
 
@@ -1625,7 +1705,7 @@ std::string Plan::describe(bool verbose) const
             os << " direct=" << (p.fuse_first ? "first" : "") << (p.fuse_first && p.fuse_last ? "," : "")
                << (p.fuse_last ? "last" : "");
         if (p.light && p.micro_count) os << " light";
-        if (p.scatter) os << " scatter";
+        if (p.scatter) os << ((long)i == scatter_pass ? " scatter" : " relabel");
         // state bits known zero at this pass's input in a run from |0...0>
         // (its loads skip those amplitudes)
         if (i < pass_zero.size() && pass_zero[i]) os << " zero_in=" << __builtin_popcountll(pass_zero[i]);

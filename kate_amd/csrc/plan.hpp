@@ -100,9 +100,9 @@ struct PassArgs {
     // without a right diagonal, dense 1-qubit ops, phases, Hadamards,
     // CNOTs): the pass runs on the kernel variant that dispatches only those.
     uint8_t light;
-    // An out-of-place permutation pass (Plan::scatter): the tile is read from
-    // one state buffer and its amplitudes written to arbitrary 2^low-amplitude
-    // runs of the other.
+    // An out-of-place pass: the tile is read from one state buffer and its
+    // amplitudes written to 2^low-amplitude runs of the other, through
+    // Plan::scatter_args[scatter_idx].
     uint8_t scatter;
     uint64_t first_g[16];
     uint64_t last_g[16];
@@ -110,6 +110,8 @@ struct PassArgs {
     // Set per launch in a run from |0...0>: amplitudes whose state offset
     // meets zero_mask are known to be zero and are not read.
     uint64_t zero_mask;
+    uint32_t scatter_idx;
+    uint32_t scatter_pad;
 };
 
 // Store side of a scatter pass: store index bit k of a tile goes to state
@@ -163,6 +165,7 @@ struct PlanOptions {
     bool split = true;       // dense 2-qubit gates that are CNOT x (A (x) B) products run as their factors
     bool reorder = true;     // swap-free programs: also plan the gates in a canonical order, keep the cheaper
     bool scatter = false;    // 2+ trailing permutation passes become one out-of-place pass (needs a second buffer)
+    bool relabel = false;    // with scatter: passes after the support region read contiguous tiles (relabelled qubits)
 };
 
 struct Plan {
@@ -191,10 +194,14 @@ struct Plan {
     // can read the state through it instead of running them.
     size_t trail_begin = 0;
     std::vector<uint64_t> trail_inv;
-    // The out-of-place permutation pass, when the trailing passes were
-    // merged into one (its index, -1 if none, and its store offsets).
+    // The pass that applies the trailing permutation out of place (-1 if
+    // none), and the store offsets of every out-of-place pass.
     long scatter_pass = -1;
-    ScatterArgs scatter_args{};
+    std::vector<ScatterArgs> scatter_args;
+    // First pass of the relabelled region: from there every pass loads
+    // its tile from state bits 0..tile_bits-1 and stores, out of place, to
+    // the positions the next pass reads (the last to virtual order).
+    long relabel_begin = -1;
     std::string describe(bool verbose = false) const;
 };
 

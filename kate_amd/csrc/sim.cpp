@@ -376,7 +376,7 @@ std::string plan_options(const kate_options& opt, kate::PlanOptions& po)
     po.amp_bytes = opt.dtype == KATE_C128 ? 16 : 8;
     if (opt.plan_flags &
         ~(KATE_PLAN_NO_EULER | KATE_PLAN_NO_DIRECT | KATE_PLAN_NO_SUPPORT | KATE_PLAN_NO_DEFER | KATE_PLAN_NO_SPLIT |
-          KATE_PLAN_NO_REORDER | KATE_PLAN_NO_FACTOR | KATE_PLAN_NO_SCATTER))
+          KATE_PLAN_NO_REORDER | KATE_PLAN_NO_FACTOR | KATE_PLAN_NO_SCATTER | KATE_PLAN_NO_RELABEL))
         return "unknown plan_flags bits";
     po.split = !(opt.plan_flags & KATE_PLAN_NO_SPLIT);
     po.reorder = !(opt.plan_flags & KATE_PLAN_NO_REORDER);
@@ -384,6 +384,7 @@ std::string plan_options(const kate_options& opt, kate::PlanOptions& po)
     po.euler = !(opt.plan_flags & KATE_PLAN_NO_EULER);
     po.factor = !(opt.plan_flags & KATE_PLAN_NO_FACTOR);
     po.scatter = !(opt.plan_flags & KATE_PLAN_NO_SCATTER);
+    po.relabel = !(opt.plan_flags & KATE_PLAN_NO_RELABEL);
     po.direct = !(opt.plan_flags & KATE_PLAN_NO_DIRECT);
     po.support = !(opt.plan_flags & KATE_PLAN_NO_SUPPORT);
     if (po.micro_bits != 3 && po.micro_bits != 4) return "micro_bits must be 3 or 4";
@@ -538,12 +539,12 @@ int kate_sim_create(const kate_program* program, const kate_options* options, ka
     step(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
     for (auto& e : s->ev) step(hipEventCreate(&e), "hipEventCreate");
     step(hipMalloc(&s->d_state, (1ull << s->nq_int) * s->amp_bytes), "hipMalloc(state)");
-    if (s->plan.scatter_pass >= 0) {
+    if (!s->plan.scatter_args.empty()) {
+        const size_t sb = s->plan.scatter_args.size() * sizeof(kate::ScatterArgs);
         step(hipMalloc(&s->d_state2, (1ull << s->nq_int) * s->amp_bytes), "hipMalloc(state 2)");
-        step(hipMalloc((void**)&s->d_scatter, sizeof(kate::ScatterArgs)), "hipMalloc(scatter)");
+        step(hipMalloc((void**)&s->d_scatter, sb), "hipMalloc(scatter)");
         if (rc == KATE_OK)
-            step(hipMemcpy(s->d_scatter, &s->plan.scatter_args, sizeof(kate::ScatterArgs), hipMemcpyHostToDevice),
-                 "upload(scatter)");
+            step(hipMemcpy(s->d_scatter, s->plan.scatter_args.data(), sb, hipMemcpyHostToDevice), "upload(scatter)");
     }
     step(hipMalloc((void**)&s->d_scratch, kate::reduce_scratch_doubles(s->nq_int) * sizeof(double)), "hipMalloc(scratch)");
     step(hipMalloc((void**)&s->d_out, (2 + 2 * (size_t)s->nq_int + 64) * sizeof(double)), "hipMalloc(out)");
