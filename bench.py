@@ -31,11 +31,11 @@ HBM_PEAK_GBS = 8000.0
 # execute() that read and write every amplitude), from rocprofv3 FETCH_SIZE
 # and WRITE_SIZE runs of `bench.py --passes-only execute`, each scaled by what
 # the same counter reads on the k_copy_tile launches of that run (16 GiB read
-# + 16 GiB written per launch: factors 2.000 and 1.000): 1.0003 x the
+# + 16 GiB written per launch: factors 2.000 and 1.000): 1.00002 x the
 # algorithmic bytes (profiles/k_pass_f64_30q_layered_hbm_counters.md).  The
 # ratio holds for the plan it was measured on; the bench reports it only when
 # the current plan of the default workload is that plan.
-TRAFFIC_RATIO_30Q_LAYERED = 1.0003
+TRAFFIC_RATIO_30Q_LAYERED = 1.00002
 TRAFFIC_PLAN_30Q_LAYERED = "nq_int=30 tile_bits=12 low=3 passes=24 (gate 24, remap 0) micro=151 ops=199"
 METRIC = "30q circuit ms + HBM % peak; served circuits/s + p99 at fidelity≥1-1e-6"
 
@@ -208,8 +208,9 @@ def main():
             copy_ms = None
             if args.passes_only == "execute":
                 _, ms, tiles = sim.execute_timed()
-                # the same passes' tiles copied in place (k_copy_tile): the
-                # HBM time each pass's access pattern allows
+                # the same passes' tiles copied with no arithmetic and each
+                # pass's own addressing (k_copy_tile in place, k_scatter for
+                # out-of-place passes): the HBM time each pattern allows
                 copy_ms = sim.copy_ceiling()
             else:
                 sim.init_basis(0)
@@ -269,10 +270,12 @@ def main():
     dense_mean = float(dense_ms.mean())
     log(f"fp64 dense (support skipping off): {ms_dense:.2f} ms/circuit, mean pass {dense_mean:.3f} ms")
     # (the scatter store folds those passes into the last gate pass; the
-    # deferral is measured on the plan that keeps them as passes)
+    # deferral is measured on the plan that keeps them as passes, which is
+    # also in place: relabelled passes need the same second buffer)
     with K.Simulator(K.Circuit(n, gates), dtype="c128", defer=True, scatter=False) as deferred:
         ms_deferred = time_executes(deferred, args.steps, args.warmup)
-    log(f"fp64 with the trailing permutation passes kept as passes and deferred: {ms_deferred:.2f} ms/circuit")
+    log(f"fp64 in place with the trailing permutation passes kept as passes and deferred: "
+        f"{ms_deferred:.2f} ms/circuit")
 
     out = {
         "metric": METRIC, "value": round(ms64, 3), "unit": "ms", "n_gpus": 1, "steps": args.steps,
@@ -291,8 +294,9 @@ def main():
                      "copy_ceiling": {"mean_launch_ms": round(ceil_full, 4),
                                       "achieved": round(bpp / (ceil_full * 1e-3) / 1e9, 1),
                                       "frac_of_ceiling": round(ceil_full / mean_pass, 4),
-                                      "what": "the same passes' tiles copied in place with no arithmetic "
-                                              "(kate_sim_copy_ceiling): the HBM rate their access pattern allows"},
+                                      "what": "the same passes' tiles copied with each pass's own addressing and "
+                                              "no arithmetic (kate_sim_copy_ceiling): the HBM rate their access "
+                                              "pattern allows"},
                      "compute": {"bound": "fp64 valu", "achieved": round(tflops, 2),
                                  "peak": FP64_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
                                  "frac": round(tflops / FP64_VALU_PEAK_TFLOPS, 4),

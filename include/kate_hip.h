@@ -145,8 +145,8 @@ enum {
        the first pass after which no qubit is known zero, reading each tile
        from the lowest state bits and writing (out of place, into the second
        buffer) where the next pass reads (the last pass to virtual order;
-       needs the second buffer as KATE_PLAN_NO_SCATTER describes; results bit
-       for bit the same) */
+       plans of 12-qubit tiles or narrower; needs the second buffer as
+       KATE_PLAN_NO_SCATTER describes; results bit for bit the same) */
     KATE_PLAN_NO_RELABEL = 256
 };
 
@@ -246,10 +246,13 @@ int kate_sim_stream(kate_sim* sim, void** stream);
  * (HIPcode/simulator.hpp:322-351). */
 int kate_sim_run_timed(kate_sim* sim, float* pass_ms, int32_t* pass_is_remap, int32_t cap, int32_t* npasses);
 /* Calibration for the passes' roofline (no counterpart in Kate): for each
- * pass of the plan, the time of a plain in-place copy of every tile with the
- * pass's tile positions (each amplitude read once and written once, no
- * arithmetic, no LDS), i.e. the HBM time the pass's access pattern allows.
- * The state is left unchanged.  pass_ms[i] for i < cap; *npasses passes. */
+ * pass of the plan, the time of a plain copy of every tile with the pass's
+ * own addressing (each amplitude read once and written once, no
+ * arithmetic): in place with the pass's tile positions (no LDS), or, for a
+ * pass that writes out of place (scatter or relabelled), through LDS and
+ * the same store into the second buffer, which does not become the state;
+ * i.e. the HBM time the pass's access pattern allows.  The state is left
+ * unchanged.  pass_ms[i] for i < cap; *npasses passes. */
 int kate_sim_copy_ceiling(kate_sim* sim, float* pass_ms, int32_t cap, int32_t* npasses);
 /* kate_sim_execute with a HIP event after every pass on the simulator's
  * stream: pass_ms[i] is pass i's time and pass_tiles[i] the number of tiles

@@ -272,7 +272,7 @@ def test_scatter_pass_every_read_path(gpu, dtype):
 
 RELABEL_CASES = [
     ("layered", 20, "c128", 0, lambda n: W.layered(n, 8, 61)),
-    ("layered", 20, "c64", 0, lambda n: W.layered(n, 8, 61)),
+    ("layered", 20, "c64", 12, lambda n: W.layered(n, 8, 61)),
     ("brick", 18, "c128", 0, lambda n: W.brickwork(n, 10, 62, longrange_offset=9)),
     ("brick", 18, "c64", 12, lambda n: W.brickwork(n, 10, 62, longrange_offset=9)),
     ("mixed", 16, "c128", 10, lambda n: W.random_mixed(n, 250, 63, max_dense=4) + W.layered(n, 2, 64)),
