@@ -1441,10 +1441,10 @@ static void make_scatter(Plan& plan, int amp_bytes)
 // amplitudes where the next pass's tile and rest qubits are placed (the last
 // one to virtual order, or through the trailing permutation).  Only physical
 // positions change; the arithmetic is that of the plain plan, bit for bit.
-// 30 qubits depth 20: complex128 12-qubit tiles 182 -> 163 ms, complex64
-// 12-qubit tiles 105 -> 98 ms; complex64 13-qubit tiles 96.1 -> 97.1 ms
-// (their passes are bound by the register steps, not by the reads), so
-// tiles wider than 12 qubits stay in place.
+// 30 qubits depth 20, 12-qubit tiles with 3 low qubits: complex128
+// 182 -> 163 ms, complex64 105 -> 98 ms.  Since the reads no longer depend
+// on the low qubits, plans with fewer of them (fewer passes, shorter write
+// runs) become the cheaper ones; see plan_cost.
 static uint64_t permute_bits(const std::vector<int>& to, uint64_t x)
 {
     uint64_t y = 0;
@@ -1460,7 +1460,7 @@ static void make_relabel(Plan& plan)
 {
     const int n = plan.nq_int, t = plan.tile_bits;
     const size_t np = plan.passes.size();
-    if (n > 64 || t > 12 || plan.remap_passes || np < 3 || plan.pass_zero.size() < np) return;
+    if (n > 64 || plan.remap_passes || np < 3 || plan.pass_zero.size() < np) return;
     size_t start = np;
     while (start > 0 && plan.pass_zero[start - 1] == 0) start--;
     if (start == 0 || start + 2 > np) return;
@@ -1560,12 +1560,35 @@ static Program canonical_order(const Program& in)
     return out;
 }
 
-// Relative cost of a plan: passes x the HBM cost of its run length, plus a
-// tenth of a pass per micro-step.
+// Relative HBM time of a relabelled pass, which reads whole tile blocks and
+// writes runs of `run_bytes`, in the units of run_cost.  Fitted on one
+// MI355X, 30-qubit layered circuit: relabelled passes of s register steps
+// take about 0.5 + 0.1 s (complex128) and 0.67 + 0.11 s (complex64) times
+// an in-place 128 B-run copy, for 64 B write runs as for longer ones; 32 B
+// runs double some passes.
+static double relabel_run_cost(int run_bytes)
+{
+    return run_bytes <= 32 ? 2.5 : 0.6;
+}
+
+// Relative cost of a plan: per pass the HBM cost of its run length (in
+// place: reads and writes; relabelled: the writes only) times the fraction
+// of its tiles that a run from |0...0> launches (a tile whose rest qubits
+// include one still known zero is skipped), plus a tenth of a pass per
+// micro-step.
 static double plan_cost(const Plan& p, int amp_bytes)
 {
     const int run = (1 << std::min(p.low_qubits, p.nq_int)) * amp_bytes;
-    return (double)p.passes.size() * run_cost(run) + 0.1 * (double)p.micro.size();
+    double c = 0.1 * (double)p.micro.size();
+    for (size_t i = 0; i < p.passes.size(); i++) {
+        const PassArgs& pa = p.passes[i];
+        double f = 1.0;
+        if ((int)i < p.support_passes && i < p.pass_zero.size())
+            for (int j = 0; j < pa.nrest; j++)
+                if ((p.pass_zero[i] >> pa.rest_pos[j]) & 1ull) f *= 0.5;
+        c += f * (p.relabel_begin >= 0 && (long)i >= p.relabel_begin ? relabel_run_cost(run) : run_cost(run));
+    }
+    return c;
 }
 
 std::string build_plan(const Program& prog, const PlanOptions& opt, Plan& plan)
@@ -1621,13 +1644,21 @@ std::string build_plan(const Program& prog, const PlanOptions& opt, Plan& plan)
         // not monotonic in it (28-qubit layered circuit: 32 passes with 3
         // low qubits, 28 with 4).  Plan each candidate and keep the cheapest
         // by pass count x run cost + 0.1 pass per micro-step (measured: a
-        // register step costs about a tenth of a 128 B-run pass).
+        // register step costs about a tenth of a 128 B-run pass).  With
+        // relabelled passes the reads are whole blocks whatever the low
+        // qubits, so plans with fewer low qubits and fewer passes gain
+        // (30-qubit layered circuit, complex64 13-qubit tiles: 18 passes with
+        // 3 low qubits 85.6 ms, 25 with 5 96.6 ms).  Relabelled complex128
+        // plans keep at least 3 low qubits (128 B runs): with 2 the same
+        // circuit takes 19 passes and 158 ms against 24 passes and 162 ms,
+        // but each pass moves its 34.4 GB in 10.2 ms instead of 7.9 ms; the
+        // longer runs are kept for the passes' nearness to their HBM bound.
         PlanOptions o = opt;
         o.auto_low = false;
         std::string first_err;
         double best = 0;
         bool have = false;
-        const int lo = opt.amp_bytes == 16 ? 2 : 3;
+        const int lo = opt.amp_bytes == 16 && !(opt.scatter && opt.relabel) ? 2 : 3;
         for (int low = lo; low <= lo + 3; low++) {
             if (low > std::min(opt.tile_bits, kMaxQubits) - opt.micro_bits) break;
             o.low_qubits = low;

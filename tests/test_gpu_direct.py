@@ -279,6 +279,8 @@ RELABEL_CASES = [
     ("qft", 18, "c64", 12, lambda n: W.layered(n, 2, 5) + W.qft(n)),
     ("haar", 17, "c64", 11, lambda n: W.haar_layers(n, 12, 65)),
     ("haar", 18, "c128", 12, lambda n: W.haar_layers(n, 12, 65)),
+    ("layered22", 22, "c128", 0, lambda n: W.layered(n, 10, 66)),
+    ("layered22", 22, "c64", 13, lambda n: W.layered(n, 10, 66)),
 ]
 
 
@@ -293,8 +295,13 @@ def test_relabelled_passes_bit_exact(gpu, name, n, dtype, tile, gen):
     gates = gen(n)
     _, _, text = K.Circuit(n, gates).plan(dtype, tile_qubits=tile)
     assert " relabel" in text
-    with K.Simulator(K.Circuit(n, gates), dtype=dtype, tile_qubits=tile) as a, \
-            K.Simulator(K.Circuit(n, gates), dtype=dtype, tile_qubits=tile, relabel=False) as b:
+    # the same plan in place: the planner's low-qubit choice depends on the
+    # relabelling, so both take the relabelled plan's
+    opt = dict(tile_qubits=int(text.split("tile_bits=")[1].split()[0]), low_qubits=int(text.split("low=")[1].split()[0]))
+    in_place = K.Circuit(n, gates).plan(dtype, relabel=False, **opt)[2]
+    assert in_place.splitlines()[0] == K.Circuit(n, gates).plan(dtype, **opt)[2].splitlines()[0]
+    with K.Simulator(K.Circuit(n, gates), dtype=dtype, **opt) as a, \
+            K.Simulator(K.Circuit(n, gates), dtype=dtype, relabel=False, **opt) as b:
         pa, pb = a.execute(), b.execute()
         sa = a.state()
         assert np.array_equal(sa, b.state())
