@@ -340,7 +340,9 @@ int kate_batch_last_ms(const kate_batch* batch, float* ms);
  * planned off the caller's thread, admitted first come first served as soon
  * as their footprint fits the free arena, and run on the stream pool so that
  * requests in flight overlap on the device.  Each returns <Z_S> for up to 15
- * real-qubit masks (FP64 accumulation, normalised). */
+ * real-qubit masks (FP64 accumulation, normalised).  A request of 22 qubits
+ * or more holds two state buffers in the arena (its passes run out of place,
+ * relabelled); a smaller one holds one. */
 typedef struct kate_server_options {
     int32_t dtype;
     int32_t device;

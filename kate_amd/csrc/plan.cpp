@@ -1463,7 +1463,7 @@ static void make_relabel(Plan& plan)
     if (n > 64 || plan.remap_passes || np < 3 || plan.pass_zero.size() < np) return;
     size_t start = np;
     while (start > 0 && plan.pass_zero[start - 1] == 0) start--;
-    if (start == 0 || start + 2 > np) return;
+    if (start + 2 > np) return;
     for (size_t k = start; k < np; k++)
         if (plan.pass_is_remap[k] || plan.passes[k].tile_bits != t || (k + 1 < np && plan.passes[k].scatter)) return;
     // every pass must read and (unless it already scatters) write each state
@@ -1687,7 +1687,7 @@ std::string build_plan(const Program& prog, const PlanOptions& opt, Plan& plan)
     if (opt.amp_bytes != 16 && opt.amp_bytes != 8) return "amplitudes have 8 or 16 bytes";
     if (opt.micro_bits != 4 && (opt.mfma || opt.mfma_fuse)) return "matrix-core steps need 4-bit micro-steps";
     Builder b{prog, opt, plan};
-    b.zero = plan.nq_int >= 64 ? ~0ull : ((1ull << plan.nq_int) - 1ull);
+    b.zero = !opt.zero_start ? 0ull : plan.nq_int >= 64 ? ~0ull : ((1ull << plan.nq_int) - 1ull);
     std::vector<int> seg;
     size_t grp = 0;
     int first = 0;

@@ -166,6 +166,7 @@ struct PlanOptions {
     bool reorder = true;     // swap-free programs: also plan the gates in a canonical order, keep the cheaper
     bool scatter = false;    // 2+ trailing permutation passes become one out-of-place pass (needs a second buffer)
     bool relabel = false;    // with scatter: passes after the support region read contiguous tiles (relabelled qubits)
+    bool zero_start = true;  // the first pass sees |0...0> (false: any state, e.g. a product input; no support region)
 };
 
 struct Plan {

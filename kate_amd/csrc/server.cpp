@@ -101,7 +101,7 @@ struct Request {
     // arena ranges
     uint64_t off = 0, bytes = 0;
     uint64_t o_state = 0, o_micro = 0, o_ops = 0, o_coef = 0, o_raw = 0, o_scratch = 0, o_out = 0, o_masks = 0,
-             o_factors = 0;
+             o_factors = 0, o_state2 = 0, o_scatter = 0;
     std::vector<float> coef32;
     int stream = -1;
     hipEvent_t done_ev = nullptr;
@@ -159,6 +159,10 @@ struct kate_server {
 
 namespace {
 
+// Requests of this many qubits or more get a second state buffer and
+// out-of-place (relabelled) passes; smaller ones keep one buffer.
+constexpr int kServeOutOfPlaceQubits = 22;
+
 uint64_t round256(uint64_t x) { return (x + 255) & ~255ull; }
 
 // Footprint layout of a planned request inside its arena range.
@@ -179,6 +183,10 @@ uint64_t layout(kate_server* s, Request& r)
     r.o_out = take(64 * sizeof(double));
     r.o_masks = take(16 * sizeof(uint64_t));
     r.o_factors = take(4 * (uint64_t)r.plan.nq_int * sizeof(double));
+    if (!r.plan.scatter_args.empty()) {  // out-of-place passes: a second state buffer
+        r.o_state2 = take((1ull << r.plan.nq_int) * s->amp);
+        r.o_scatter = take(r.plan.scatter_args.size() * sizeof(kate::ScatterArgs));
+    }
     return o;
 }
 
@@ -207,6 +215,9 @@ int launch_request(kate_server* s, Request& r)
         SV(hipMemcpyAsync(base + r.o_raw, r.plan.raw.data(), r.plan.raw.size() * sizeof(uint32_t),
                           hipMemcpyHostToDevice, st));
     SV(hipMemcpyAsync(base + r.o_masks, r.masks.data(), r.masks.size() * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+    if (!r.plan.scatter_args.empty())
+        SV(hipMemcpyAsync(base + r.o_scatter, r.plan.scatter_args.data(),
+                          r.plan.scatter_args.size() * sizeof(kate::ScatterArgs), hipMemcpyHostToDevice, st));
     int k = 0;
     if (!r.product.empty()) {
         SV(hipMemcpyAsync(base + r.o_factors, r.product.data(), r.product.size() * sizeof(double),
@@ -221,10 +232,26 @@ int launch_request(kate_server* s, Request& r)
     const bool from_zero = r.product.empty();
     kate::DevPlan dp{(const kate::MicroStep*)(base + r.o_micro), (const kate::MicroOp*)(base + r.o_ops),
                      base + r.o_coef, (const uint32_t*)(base + r.o_raw)};
+    // an out-of-place pass (relabelled, or storing through the trailing
+    // permutation) writes the other buffer, which then holds the state
+    void* other = r.plan.scatter_args.empty() ? nullptr : base + r.o_state2;
+    const kate::ScatterArgs* sa = (const kate::ScatterArgs*)(base + r.o_scatter);
     for (size_t i = 0; i < r.plan.passes.size(); i++) {
         const kate::PassArgs pa = from_zero ? kate::zero_start_pass(r.plan, i, s->po.support, r.plan.passes.size()) : r.plan.passes[i];
-        k = kate::launch_pass(s->opt.dtype, r.plan.tile_bits, state, r.plan.nq_int, pa, dp, st);
+        if (!pa.scatter) {
+            k = kate::launch_pass(s->opt.dtype, r.plan.tile_bits, state, r.plan.nq_int, pa, dp, st);
+        } else if (!other) {
+            k = (int)hipErrorInvalidValue;
+        } else if (pa.micro_count) {
+            kate::DevPlan d = dp;
+            d.dst = other;
+            d.scatter = sa;
+            k = kate::launch_pass(s->opt.dtype, r.plan.tile_bits, state, r.plan.nq_int, pa, d, st);
+        } else {
+            k = kate::launch_scatter(s->opt.dtype, r.plan.tile_bits, state, other, pa, sa, st);
+        }
         if (k) return k;
+        if (pa.scatter) std::swap(state, other);
     }
     k = kate::launch_zsums(s->opt.dtype, state, r.plan.nq_int, (const uint64_t*)(base + r.o_masks),
                            (int)r.masks.size(), (double*)(base + r.o_scratch), (double*)(base + r.o_out), st);
@@ -265,7 +292,15 @@ void worker_loop(kate_server* s)
         if (!unplanned.empty()) {
             lk.unlock();
             for (auto& r : unplanned) {
-                std::string err = kate::build_plan(r->prog, s->po, r->plan);
+                // requests of kServeOutOfPlaceQubits or more run out of
+                // place (two state buffers): their passes are HBM-bound and
+                // relabelled passes read whole tile blocks
+                kate::PlanOptions po = s->po;
+                po.scatter = r->prog.nqubits >= kServeOutOfPlaceQubits;
+                // a product-state input has no support region to follow:
+                // its passes can all be relabelled
+                po.zero_start = r->theta_phi.empty();
+                std::string err = kate::build_plan(r->prog, po, r->plan);
                 if (err.empty() && s->opt.dtype == KATE_C64) r->coef32.assign(r->plan.coef.begin(), r->plan.coef.end());
                 if (err.empty() && !r->theta_phi.empty()) product_factors(*r);
                 if (!err.empty()) r->error = "planning: " + err;
@@ -380,7 +415,7 @@ int kate_server_create(const kate_server_options* options, kate_server** out)
     kate::PlanOptions po;
     std::string err = kate::plan_options(ko, po);
     if (!err.empty()) return sfail(KATE_ERR_ARG, err);
-    po.scatter = false;  // one state buffer per request
+    po.scatter = false;  // set per request, by size
     if (po.auto_tile) {
         // one tile size, planned once per request on the scheduler thread:
         // 12-qubit tiles (a mix of 128 circuits of 12..30 qubits at

@@ -107,3 +107,30 @@ def test_server_28_qubit_request_alone_equals_mixed(gpu):
         z64 = ref.expect_z(masks)
     say("28q FP32 server vs FP64 max |dZ|", float(np.max(np.abs(alone - z64))))
     assert np.max(np.abs(alone - z64)) < 2e-6
+
+
+@pytest.mark.parametrize("dtype,tol", [("c128", 1e-12), ("c64", 2e-6)])
+def test_server_out_of_place_requests(gpu, dtype, tol):
+    """Requests of 22 qubits or more run on two state buffers of their own
+    with relabelled passes (each reads its tile from the lowest state bits
+    and writes where the next pass reads): layered circuits from |0...0> and
+    QFTs on basis inputs, served beside small in-place requests, match the
+    oracle."""
+    say("server out-of-place", dtype)
+    n = 22
+    big = [(n, W.layered(n, 8, 93), None, [int(m) for m in W.zstring_masks(n, 8, 94)]),
+           (n, W.qft(n), W.basis_input(n, 0x2D5A3), [int(m) for m in W.zstring_masks(n, 8, 95)]),
+           (n + 1, W.layered(n + 1, 6, 96), None, [int(m) for m in W.zstring_masks(n + 1, 8, 97)]),
+           (n, W.qft(n), W.basis_input(n, 0x1F0F1), [int(m) for m in W.zstring_masks(n, 8, 98)])]
+    for m_, g, tp, _ in big:
+        if tp is None:  # from |0...0>: relabelled after the support region
+            assert " relabel" in K.Circuit(m_, g).plan(dtype, tile_qubits=12)[2]
+    small = make_requests(4, 92, 12, 16)
+    reqs = [r for pair in zip(big, small) for r in pair]
+    with K.Server(dtype=dtype, streams=4, max_active=8) as srv:
+        ids = [srv.submit(K.Circuit(n, g), m, tp) for n, g, tp, m in reqs]
+        outs = [srv.wait(i) for i in ids]
+    for (n, g, tp, m), (z, st) in zip(reqs, outs):
+        if n >= 22:  # a second state buffer in the request's footprint
+            assert st["footprint_bytes"] >= 2 * (1 << n) * (16 if dtype == "c128" else 8)
+        assert np.max(np.abs(z - oracle_z(n, g, tp, m))) < tol
