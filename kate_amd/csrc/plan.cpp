@@ -1586,7 +1586,10 @@ static double plan_cost(const Plan& p, int amp_bytes)
         if ((int)i < p.support_passes && i < p.pass_zero.size())
             for (int j = 0; j < pa.nrest; j++)
                 if ((p.pass_zero[i] >> pa.rest_pos[j]) & 1ull) f *= 0.5;
-        c += f * (p.relabel_begin >= 0 && (long)i >= p.relabel_begin ? relabel_run_cost(run) : run_cost(run));
+        // (a pass of the full op set is bound by its register steps; its
+        // reads cost it what they cost in place)
+        const bool relabelled = p.relabel_begin >= 0 && (long)i >= p.relabel_begin && pa.light;
+        c += f * (relabelled ? relabel_run_cost(run) : run_cost(run));
     }
     return c;
 }
