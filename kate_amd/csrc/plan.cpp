@@ -1436,8 +1436,9 @@ static void make_scatter(Plan& plan, int amp_bytes)
 // more slowly (one MI355X, complex128, 30 qubits: an out-of-place copy of
 // tiles over qubits {0,1,2,18..26} 8.3 ms, reading contiguous tiles and
 // writing to those positions 6.7 ms).  From the first pass after which no
-// qubit is still known zero (earlier passes skip tiles in place), every pass
-// is out of place: it reads its tile from bits 0..tile_bits-1 and writes its
+// qubit is still known zero (earlier passes skip tiles in place; a plan for
+// any input state, PlanOptions::zero_start off, has none), every pass is
+// out of place: it reads its tile from bits 0..tile_bits-1 and writes its
 // amplitudes where the next pass's tile and rest qubits are placed (the last
 // one to virtual order, or through the trailing permutation).  Only physical
 // positions change; the arithmetic is that of the plain plan, bit for bit.
