@@ -11,14 +11,32 @@ from kate_amd import workloads as W
 from oracle import oracle as O
 from util import compiled_from_fixture, dense_unitary, fixtures, load, to_real_order
 
-SIM_FIXTURES = [p for p in fixtures() if "merge" not in p and "config1" not in p]
+# Fixtures with a Kate state.  Programs with swap instructions have none
+# (Kate's CPU simulator ignores swaps, so its state would not be the
+# program's); test_oracle_swaps_on_kate_programs pins the oracle on those.
+PROG_FIXTURES = [p for p in fixtures() if "merge" not in p and "config1" not in p]
+SIM_FIXTURES = [p for p in PROG_FIXTURES if load(p)[1]["simulated"]]
+SWAP_FIXTURES = [p for p in PROG_FIXTURES if not load(p)[1]["simulated"]]
+
+
+@pytest.mark.parametrize("path", SWAP_FIXTURES, ids=lambda p: p.split("/")[-1][5:-4])
+def test_oracle_swaps_on_kate_programs(path):
+    """Programs Kate's compiler emitted with swap instructions (grouped for
+    and allocated over 4 GPUs): the oracle's state, read in real qubit order
+    through the program's final_inverse_permutation, is the swap-free
+    simulation of the source circuit (compilation only reorders, fuses and
+    relabels, so the two agree to rounding)."""
+    z, meta = load(path)
+    prog = compiled_from_fixture(z, meta)
+    assert any(k == 0 for k, _ in prog.instructions)
+    st = to_real_order(O.run(prog), prog.final_inverse_permutation)
+    ref = O.simulate(meta["n"], unpack_gates(z, "in_"))
+    assert np.max(np.abs(st - ref)) <= 1e-14
 
 
 @pytest.mark.parametrize("path", SIM_FIXTURES, ids=lambda p: p.split("/")[-1][5:-4])
 def test_oracle_equals_kate_fixture(path):
     z, meta = load(path)
-    if not meta["simulated"]:
-        pytest.skip("program has swap instructions; Kate's CPU simulator ignores them")
     prog = compiled_from_fixture(z, meta)
     pin = z["proba_in"] if "proba_in" in z else None
     st = O.run(prog, pin)
