@@ -394,6 +394,65 @@ __device__ __forceinline__ void generic_dense(Cx<R>* tile, const MicroStep* __re
     for (int e = 0; e < N; e++) tile[swz<R>(layout<TB>((uint32_t)e * NT + threadIdx.x, ms.cols))] = out[e];
 }
 
+// One dense 5-qubit gate on the LDS tile (an SK_GENERIC step of 5 bits):
+// the tile splits into 2^(TB-5) groups of 32 amplitudes that the gate mixes.
+// Lane l of a wave holds group l (mod the group count) and all threads of a
+// wave compute the same rows, so the matrix entries are wave-uniform and
+// come in through scalar loads; each thread reads its group's 32 amplitudes
+// into registers and, after a barrier, writes its rows.  The products and
+// their order are generic_dense's, so the results are bit for bit the same.
+template <typename R, int TB, int NB>
+__device__ __forceinline__ void dense5(Cx<R>* tile, const MicroStep* __restrict__ msp,
+                                       const Cx<R>* __restrict__ m)
+{
+    constexpr int NT = 1 << (TB - NB);  // threads of the workgroup
+    constexpr int G = 1 << (TB - 5);    // groups of 32 amplitudes
+    constexpr int RPT = 32 * G / NT;    // rows per thread
+    static_assert(NT % G == 0 && RPT >= 1, "each group needs a whole number of threads");
+    const uint32_t tid = threadIdx.x;
+    const uint32_t grp = tid % (uint32_t)G;
+    uint32_t part = tid / (uint32_t)G;
+    if constexpr (G >= 64) part = __builtin_amdgcn_readfirstlane(part);  // one part per wave
+    uint32_t gmask = 0, oc[5];
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+        gmask |= 1u << msp->bits[i];
+        oc[i] = msp->cols[msp->bits[i]];
+    }
+    // the group's tile index: its bits on the non-gate tile bits, ascending
+    uint32_t y = 0, g = grp;
+#pragma unroll
+    for (int b = 0; b < TB; b++)
+        if (!((gmask >> b) & 1u)) {
+            y |= (g & 1u) << b;
+            g >>= 1;
+        }
+    const uint32_t base = layout<TB>(y, msp->cols);
+    Cx<R> x[32];
+#pragma unroll
+    for (int c = 0; c < 32; c++) {
+        uint32_t idx = base;
+#pragma unroll
+        for (int i = 0; i < 5; i++)
+            if ((c >> i) & 1) idx ^= oc[i];
+        x[c] = tile[swz<R>(idx)];
+    }
+    __syncthreads();
+    const uint32_t row0 = part * (uint32_t)RPT;
+    for (int r = 0; r < RPT; r++) {
+        const uint32_t row = row0 + (uint32_t)r;
+        const Cx<R>* __restrict__ mr = m + row * 32u;
+        Cx<R> acc{0, 0};
+#pragma unroll
+        for (int c = 0; c < 32; c++) acc = cmac(acc, mr[c], x[c]);
+        uint32_t idx = base;
+#pragma unroll
+        for (int i = 0; i < 5; i++)
+            if ((row >> i) & 1u) idx ^= oc[i];
+        tile[swz<R>(idx)] = acc;
+    }
+}
+
 // WIDE instantiations also run SK_GENERIC micro-steps; keeping them out of
 // the common variant keeps its register count at two waves per SIMD.
 typedef float f32x16 __attribute__((ext_vector_type(16)));
@@ -719,7 +778,12 @@ __device__ __forceinline__ void run_steps(const PassArgs& pa, Cx<R>* tile, const
 #pragma unroll
             for (int s = 0; s < N; s++) lds_at(tile, bw ^ reg_offset<NB>(s, oc)) = v[s];
         } else if (WIDE) {
-            if constexpr (sizeof(R) == 4 && NB == 4) {
+            // (a 1024-thread workgroup is held to 128 VGPRs: too few for
+            // dense5's 32 amplitudes in complex128)
+            constexpr bool kDense5 = sizeof(R) == 4 || (TB - NB) <= 9;
+            if (kDense5 && msp->kind == SK_GENERIC && msp->nbits == 5) {
+                dense5<R, TB, NB>(tile, msp, coef + msp->coef);
+            } else if constexpr (sizeof(R) == 4 && NB == 4) {
                 Cx<float>* tf = reinterpret_cast<Cx<float>*>(tile);
                 if (msp->kind == SK_MFMA) {
                     const float* A = reinterpret_cast<const float*>(raw + msp->coef);

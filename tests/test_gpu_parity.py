@@ -405,3 +405,27 @@ def test_factored_steps_in_swap_programs(gpu, dtype):
         assert np.max(np.abs(st_real - st0)) <= 1e-12
     else:
         assert 1.0 - fidelity(st_real, st0) <= FID32
+
+
+def dense_gates(n, count, k, seed):
+    rng = np.random.default_rng(seed)
+    return [W.Gate(W.DENSE, tuple(int(x) for x in rng.choice(n, size=k, replace=False)), 0,
+                   W.haar_unitary(rng, 1 << k)) for _ in range(count)]
+
+
+@pytest.mark.parametrize("dtype,tile,mb", [("c128", 12, 4), ("c128", 10, 4), ("c128", 11, 3), ("c128", 12, 3),
+                                           ("c64", 12, 4), ("c64", 13, 4), ("c64", 10, 4)])
+def test_dense_five_qubit_gates(gpu, dtype, tile, mb):
+    """Dense 5-qubit gates (register-blocked LDS steps: each thread holds a
+    group of 32 amplitudes, the matrix rows are wave-uniform) against the
+    oracle, on every tile width and both register widths, with 1- to
+    4-qubit gates between them."""
+    say("dense5", dtype, tile, mb)
+    n = 16
+    gates = dense_gates(n, 24, 5, 70 + tile) + W.random_mixed(n, 60, 71, max_dense=4) + dense_gates(n, 12, 5, 72)
+    st, _ = gpu_state(K.Circuit(n, gates), dtype=dtype, tile_qubits=tile, micro_bits=mb)
+    ref = O.simulate(n, gates)
+    if dtype == "c128":
+        assert np.max(np.abs(st - ref)) <= TOL64
+    else:
+        assert 1.0 - abs(np.vdot(ref, st.astype(np.complex128))) ** 2 <= 1e-6
