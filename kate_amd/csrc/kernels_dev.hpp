@@ -439,6 +439,7 @@ __device__ __forceinline__ void dense5(Cx<R>* tile, const MicroStep* __restrict_
     }
     __syncthreads();
     const uint32_t row0 = part * (uint32_t)RPT;
+#pragma unroll 2
     for (int r = 0; r < RPT; r++) {
         const uint32_t row = row0 + (uint32_t)r;
         const Cx<R>* __restrict__ mr = m + row * 32u;
