@@ -147,7 +147,11 @@ enum {
        buffer) where the next pass reads (the last pass to virtual order;
        plans of 12-qubit tiles or narrower; needs the second buffer as
        KATE_PLAN_NO_SCATTER describes; results bit for bit the same) */
-    KATE_PLAN_NO_RELABEL = 256
+    KATE_PLAN_NO_RELABEL = 256,
+    /* keep dense 5-qubit gates of a KATE_C64 plan on the VALU (register
+       blocks) instead of the FP32 matrix cores, which take them by default
+       (use_mfma = 0); results to FP32 rounding */
+    KATE_PLAN_NO_MATRIX5 = 512
 };
 
 /* Wall-clock split of the last execute, as Kate's execute(true) prints it

@@ -44,14 +44,17 @@ PLAN_NO_EULER, PLAN_NO_DIRECT, PLAN_NO_SUPPORT, PLAN_NO_DEFER, PLAN_NO_SPLIT, PL
 PLAN_NO_FACTOR = 64
 PLAN_NO_SCATTER = 128
 PLAN_NO_RELABEL = 256
+PLAN_NO_MATRIX5 = 512
 
 
 def _plan_flags(euler: bool, direct: bool, support: bool = True, defer: bool = True, split: bool = True,
-                reorder: bool = True, factor: bool = True, scatter: bool = True, relabel: bool = True) -> int:
+                reorder: bool = True, factor: bool = True, scatter: bool = True, relabel: bool = True,
+                matrix5: bool = True) -> int:
     return ((0 if euler else PLAN_NO_EULER) | (0 if direct else PLAN_NO_DIRECT) | (0 if support else PLAN_NO_SUPPORT)
             | (0 if defer else PLAN_NO_DEFER) | (0 if split else PLAN_NO_SPLIT)
             | (0 if reorder else PLAN_NO_REORDER) | (0 if factor else PLAN_NO_FACTOR)
-            | (0 if scatter else PLAN_NO_SCATTER) | (0 if relabel else PLAN_NO_RELABEL))
+            | (0 if scatter else PLAN_NO_SCATTER) | (0 if relabel else PLAN_NO_RELABEL)
+            | (0 if matrix5 else PLAN_NO_MATRIX5))
 
 
 class Timings(C.Structure):
@@ -291,13 +294,13 @@ class Circuit:
     def plan(self, dtype: str = "c128", tile_qubits: int = 0, low_qubits: int = 0, mfma: int = 0,
              micro_bits: int = 0, euler: bool = True, direct: bool = True,
              split: bool = True, reorder: bool = True, factor: bool = True,
-             scatter: bool = True, relabel: bool = True) -> Tuple[int, int, str]:
+             scatter: bool = True, relabel: bool = True, matrix5: bool = True) -> Tuple[int, int, str]:
         """(gate passes, remap passes, text) of the device plan, without a GPU
         (a simulator plans the scatter pass only when the GPU has room for
         a second state buffer)."""
         opt = Options({"c128": KATE_C128, "c64": KATE_C64}[dtype], tile_qubits, low_qubits, mfma, 0, 0, micro_bits,
                       _plan_flags(euler, direct, split=split, reorder=reorder, factor=factor, scatter=scatter,
-                                  relabel=relabel))
+                                  relabel=relabel, matrix5=matrix5))
         p, r = C.c_int32(), C.c_int32()
         buf = C.create_string_buffer(1 << 20)
         _check(lib().kate_circuit_plan(self._h, C.byref(opt), C.byref(p), C.byref(r), buf, 1 << 20), "plan")
@@ -320,7 +323,7 @@ class Simulator:
     def __init__(self, circuit: Circuit, dtype: str = "c128", tile_qubits: int = 0, low_qubits: int = 0,
                  device: int = 0, mfma: int = 0, micro_bits: int = 0, euler: bool = True, direct: bool = True,
                  support: bool = True, defer: bool = False, split: bool = True, reorder: bool = True,
-                 factor: bool = True, scatter: bool = True, relabel: bool = True):
+                 factor: bool = True, scatter: bool = True, relabel: bool = True, matrix5: bool = True):
         """`defer=True` lets execute() leave the trailing passes that only
         permute amplitudes (CNOT layers at the end of the circuit) unrun:
         its proba_state reads through them and any later read of the state
@@ -331,12 +334,14 @@ class Simulator:
         `factor=False` keeps Euler-form Kronecker steps unscaled;
         `scatter=False` runs the trailing permutation passes as planned
         instead of as one out-of-place pass; `relabel=False` keeps every
-        pass in place on the virtual qubit order."""
+        pass in place on the virtual qubit order; `matrix5=False` keeps dense
+        5-qubit gates of a complex64 state on the VALU instead of the matrix
+        cores."""
         self.n = circuit.n
         self.dtype = {"c128": KATE_C128, "c64": KATE_C64}[dtype]
         self.amp = np.complex128 if self.dtype == KATE_C128 else np.complex64
         opt = Options(self.dtype, tile_qubits, low_qubits, mfma, 0, device, micro_bits,
-                      _plan_flags(euler, direct, support, defer, split, reorder, factor, scatter, relabel))
+                      _plan_flags(euler, direct, support, defer, split, reorder, factor, scatter, relabel, matrix5))
         h = VP()
         _check(lib().kate_sim_create_from_circuit(circuit._h, C.byref(opt), C.byref(h)), "kate_sim_create")
         self._h = h

@@ -141,6 +141,7 @@ int kate_batch_create(const kate_program* programs, int32_t count, const kate_op
     po.factor = false;
     po.auto_tile = false;
     po.scatter = false;
+    po.mfma5 = false;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return bfail(KATE_ERR_NO_DEVICE, "no GPU visible");
 

@@ -946,7 +946,8 @@ struct Builder {
                 // the first remaining gate is one the register path does not take
                 const PlanGate& g = prog.gates[(size_t)rem[0]];
                 if (g.id != 0) return "internal: non-dense gate left out of micro-steps";
-                ms.kind = (opt.mfma && g.k >= 4 && g.k <= 5) ? (opt.mfma_bf16 ? SK_MFMA_BF16 : SK_MFMA) : SK_GENERIC;
+                const bool matrix = (opt.mfma && g.k >= 4 && g.k <= 5) || (opt.mfma5 && g.k == 5);
+                ms.kind = matrix ? (opt.mfma_bf16 ? SK_MFMA_BF16 : SK_MFMA) : SK_GENERIC;
                 ms.nbits = (uint8_t)g.k;
                 for (int i = 0; i < g.k; i++) ms.bits[i] = (uint8_t)tile_of[g.q[i]];
                 if (ms.kind == SK_MFMA || ms.kind == SK_MFMA_BF16) {
@@ -1689,7 +1690,7 @@ std::string build_plan(const Program& prog, const PlanOptions& opt, Plan& plan)
     plan.tile_bits = std::min(opt.tile_bits, plan.nq_int);
     if (opt.micro_bits != 3 && opt.micro_bits != 4) return "micro-steps have 3 or 4 bits";
     if (opt.amp_bytes != 16 && opt.amp_bytes != 8) return "amplitudes have 8 or 16 bytes";
-    if (opt.micro_bits != 4 && (opt.mfma || opt.mfma_fuse)) return "matrix-core steps need 4-bit micro-steps";
+    if (opt.micro_bits != 4 && (opt.mfma || opt.mfma_fuse || opt.mfma5)) return "matrix-core steps need 4-bit micro-steps";
     Builder b{prog, opt, plan};
     b.zero = !opt.zero_start ? 0ull : plan.nq_int >= 64 ? ~0ull : ((1ull << plan.nq_int) - 1ull);
     std::vector<int> seg;

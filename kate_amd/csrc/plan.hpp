@@ -151,6 +151,7 @@ struct PlanOptions {
     int low_qubits = 3;   // qubits 0..low-1 are in every gate tile
     int min_qubits = 12;  // the state is padded to at least this many qubits
     bool mfma = false;       // SK_MFMA for dense gates of 4..5 qubits (complex64 only)
+    bool mfma5 = false;      // SK_MFMA for dense gates of 5 qubits only (complex64 only)
     bool mfma_fuse = false;  // also compose the gates of 5-qubit windows into SK_MFMA steps
     bool mfma_bf16 = false;  // matrix-core steps use split-bf16 products (SK_MFMA_BF16)
     int micro_bits = 4;      // register micro-steps span 3 or 4 tile bits (8 or 16 amplitudes per thread)

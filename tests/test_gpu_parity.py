@@ -413,17 +413,19 @@ def dense_gates(n, count, k, seed):
                    W.haar_unitary(rng, 1 << k)) for _ in range(count)]
 
 
-@pytest.mark.parametrize("dtype,tile,mb", [("c128", 12, 4), ("c128", 10, 4), ("c128", 11, 3), ("c128", 12, 3),
-                                           ("c64", 12, 4), ("c64", 13, 4), ("c64", 10, 4)])
-def test_dense_five_qubit_gates(gpu, dtype, tile, mb):
-    """Dense 5-qubit gates (register-blocked LDS steps: each thread holds a
-    group of 32 amplitudes, the matrix rows are wave-uniform) against the
-    oracle, on every tile width and both register widths, with 1- to
-    4-qubit gates between them."""
-    say("dense5", dtype, tile, mb)
+@pytest.mark.parametrize("dtype,tile,mb,matrix5", [("c128", 12, 4, True), ("c128", 10, 4, True), ("c128", 11, 3, True),
+                                                   ("c128", 12, 3, True), ("c64", 12, 4, False), ("c64", 13, 4, False),
+                                                   ("c64", 10, 4, False), ("c64", 12, 4, True), ("c64", 13, 4, True),
+                                                   ("c64", 10, 4, True)])
+def test_dense_five_qubit_gates(gpu, dtype, tile, mb, matrix5):
+    """Dense 5-qubit gates against the oracle, on every tile width and both
+    register widths, with 1- to 4-qubit gates between them: register-blocked
+    LDS steps (each thread holds a group of 32 amplitudes, the matrix rows
+    are wave-uniform), and for complex64 by default the FP32 matrix cores."""
+    say("dense5", dtype, tile, mb, matrix5)
     n = 16
     gates = dense_gates(n, 24, 5, 70 + tile) + W.random_mixed(n, 60, 71, max_dense=4) + dense_gates(n, 12, 5, 72)
-    st, _ = gpu_state(K.Circuit(n, gates), dtype=dtype, tile_qubits=tile, micro_bits=mb)
+    st, _ = gpu_state(K.Circuit(n, gates), dtype=dtype, tile_qubits=tile, micro_bits=mb, matrix5=matrix5)
     ref = O.simulate(n, gates)
     if dtype == "c128":
         assert np.max(np.abs(st - ref)) <= TOL64
