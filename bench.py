@@ -168,6 +168,7 @@ def main():
     ap.add_argument("--no-c2", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--no-qulacs", action="store_true")
+    ap.add_argument("--no-dense5", action="store_true")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--serve-requests", type=int, default=256)
     ap.add_argument("--serve-qps", type=float, default=16.0)
@@ -330,6 +331,9 @@ def main():
     if not args.no_c3:
         out["c3_brickwork"] = c3()
         log(f"c3: {out['c3_brickwork']}")
+    if not args.no_dense5:
+        out["dense5_28q"] = dense5()
+        log(f"dense5_28q: {out['dense5_28q']}")
     if not args.no_qulacs:
         out["qulacs25"] = qulacs()
         log(f"qulacs25: {out['qulacs25']}")
@@ -434,6 +438,32 @@ def qulacs(steps=5):
         t = sim.timings()
     return {"qubits": n, "gates": len(gates), "ms": round(ms, 3), "passes": t["passes"],
             "kate_mi200_ms": 1035.41, "speedup_vs_kate_mi200": round(1035.41 / ms, 1)}
+
+
+def dense5(steps=3):
+    """Kernel family (b) where it pays: 28 qubits, 60 Haar-random dense
+    5-qubit gates on random qubits.  FP64 (register-blocked VALU steps),
+    complex64 as planned by default (the 5-qubit gates on the FP32 matrix
+    cores) and complex64 with them on the VALU; 1-F against the FP64 state."""
+    n = 28
+    rng = np.random.default_rng(75)
+    gates = [W.Gate(W.DENSE, tuple(int(x) for x in rng.choice(n, size=5, replace=False)), 0,
+                    W.haar_unitary(rng, 32)) for _ in range(60)]
+    res = {"qubits": n, "gates": len(gates)}
+    with K.Simulator(K.Circuit(n, gates), dtype="c128") as ref:
+        res["fp64_ms"] = round(time_executes(ref, steps, 1), 3)
+        for label, m5 in (("fp32_matrix_cores", True), ("fp32_valu", False)):
+            with K.Simulator(K.Circuit(n, gates), dtype="c64", matrix5=m5) as s32:
+                ms = time_executes(s32, steps, 1)
+                os.environ["KATE_DESCRIBE_OPS"] = "1"
+                try:
+                    text = K.Circuit(n, gates).plan("c64", matrix5=m5)[2]
+                finally:
+                    os.environ.pop("KATE_DESCRIBE_OPS", None)
+                res[label] = {"ms": round(ms, 3), "speedup_vs_fp64": round(res["fp64_ms"] / ms, 3),
+                              "one_minus_fidelity": float(1.0 - s32.fidelity(ref)),
+                              "mfma_steps": sum(1 for ln in text.splitlines() if re.match(r"\s+step \d+ mfma", ln))}
+    return res
 
 
 def batched(count, check=64):
