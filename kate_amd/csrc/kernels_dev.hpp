@@ -398,9 +398,11 @@ __device__ __forceinline__ void generic_dense(Cx<R>* tile, const MicroStep* __re
 // the tile splits into 2^(TB-5) groups of 32 amplitudes that the gate mixes.
 // Lane l of a wave holds group l (mod the group count) and all threads of a
 // wave compute the same rows, so the matrix entries are wave-uniform and
-// come in through scalar loads; each thread reads its group's 32 amplitudes
-// into registers and, after a barrier, writes its rows.  The products and
-// their order are generic_dense's, so the results are bit for bit the same.
+// come in through scalar loads, by columns from the transpose that the plan
+// stores after the matrix.  Each thread streams its group's 32 amplitudes
+// from LDS into its rows' accumulators and, after a barrier, writes them.
+// Every row sums its products in generic_dense's order, so the results are
+// bit for bit the same.
 template <typename R, int TB, int NB>
 __device__ __forceinline__ void dense5(Cx<R>* tile, const MicroStep* __restrict__ msp,
                                        const Cx<R>* __restrict__ m)
@@ -428,29 +430,30 @@ __device__ __forceinline__ void dense5(Cx<R>* tile, const MicroStep* __restrict_
             g >>= 1;
         }
     const uint32_t base = layout<TB>(y, msp->cols);
-    Cx<R> x[32];
+    const uint32_t row0 = part * (uint32_t)RPT;
+    const Cx<R>* __restrict__ mt = m + 1024 + row0;  // transpose: entry (row, c) at c * 32 + row
+    Cx<R> acc[RPT];
 #pragma unroll
+    for (int r = 0; r < RPT; r++) acc[r] = Cx<R>{0, 0};
+#pragma unroll 4
     for (int c = 0; c < 32; c++) {
         uint32_t idx = base;
 #pragma unroll
         for (int i = 0; i < 5; i++)
             if ((c >> i) & 1) idx ^= oc[i];
-        x[c] = tile[swz<R>(idx)];
+        const Cx<R> x = tile[swz<R>(idx)];
+#pragma unroll
+        for (int r = 0; r < RPT; r++) acc[r] = cmac(acc[r], mt[c * 32 + r], x);
     }
     __syncthreads();
-    const uint32_t row0 = part * (uint32_t)RPT;
-#pragma unroll 2
+#pragma unroll
     for (int r = 0; r < RPT; r++) {
         const uint32_t row = row0 + (uint32_t)r;
-        const Cx<R>* __restrict__ mr = m + row * 32u;
-        Cx<R> acc{0, 0};
-#pragma unroll
-        for (int c = 0; c < 32; c++) acc = cmac(acc, mr[c], x[c]);
         uint32_t idx = base;
 #pragma unroll
         for (int i = 0; i < 5; i++)
             if ((row >> i) & 1u) idx ^= oc[i];
-        tile[swz<R>(idx)] = acc;
+        tile[swz<R>(idx)] = acc[r];
     }
 }
 

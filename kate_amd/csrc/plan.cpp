@@ -956,6 +956,18 @@ struct Builder {
                     fill_mfma_cols(ms, cols);
                 } else {
                     ms.coef = add_coef(prog.mat.data() + 2 * g.moff, (size_t)1 << (2 * g.k));
+                    if (g.k == 5) {
+                        // the register-blocked 5-qubit step reads the matrix
+                        // by columns: its transpose follows the matrix
+                        const double* u = prog.mat.data() + 2 * g.moff;
+                        std::vector<double> t(2 * 1024);
+                        for (int r = 0; r < 32; r++)
+                            for (int c = 0; c < 32; c++) {
+                                t[2 * (size_t)(c * 32 + r)] = u[2 * (size_t)(r * 32 + c)];
+                                t[2 * (size_t)(c * 32 + r) + 1] = u[2 * (size_t)(r * 32 + c) + 1];
+                            }
+                        add_coef(t.data(), 1024);
+                    }
                 }
                 plan.micro.push_back(ms);
                 rem.erase(rem.begin());
